@@ -1,0 +1,188 @@
+/*
+ * scir_hip.h -- C ABI of the MI355X (gfx950) batched-FIR library
+ *               (scir_amd/libscir_hip.so).
+ *
+ * This is the drop-in boundary for SciR's batched FIR path:
+ *   scir_gpu::fir1d_batched_f32_auto(x, taps, Device)   crates/scir-gpu/src/lib.rs:515-531
+ *     -> cuda::fir1d_batched_f32_cuda(x, taps)          crates/scir-gpu/src/lib.rs:1036-1113
+ *     -> CPU fir1d_batched_f32(x, taps)                 crates/scir-gpu/src/lib.rs:1134-1152
+ * A Rust `-sys` crate binds these symbols one to one; the `Device::Hip` arm of
+ * the dispatcher calls scir_hip_fir_f32_host().
+ *
+ * Conventions shared by every entry point
+ *   - All functions return an int status (scir_hip_status); 0 is success.  On
+ *     failure scir_hip_last_error() returns a message naming the failing call
+ *     (the thread's last error), like GpuError::BackendUnavailable(String)
+ *     (crates/scir-gpu/src/lib.rs:583-589).  No C++ exception crosses the ABI.
+ *   - Taps are passed as `b` in lfilter order: y[n] = sum_k b[k] * x[n-k] with
+ *     x[<0] = 0 (zero initial state), output length == input length.  The
+ *     reference's `taps` argument is b reversed (b[k] == taps[K-1-k]); the
+ *     host-side dispatcher reverses it before the call.
+ *   - 2-D arrays are row-major with an explicit row stride in elements
+ *     (ldx >= N); rows are independent signals.
+ *   - `*_host` entry points take and return host memory and are synchronous.
+ *     `*_dev` entry points take device pointers and a hipStream_t passed as
+ *     void* (NULL = the null stream); they enqueue work and return without
+ *     synchronising.  Callers own all memory they pass in.
+ *   - Thread safety: every entry point may be called from any thread.  Work
+ *     that shares library-owned scratch (the MFMA tap fragments, the dL/db
+ *     partial sums, the host-tier staging buffers) is serialised per device.
+ */
+#ifndef SCIR_HIP_H
+#define SCIR_HIP_H
+
+#include <stdint.h>
+
+#if defined(__GNUC__) || defined(__clang__)
+#define SCIR_HIP_API __attribute__((visibility("default")))
+#else
+#define SCIR_HIP_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+typedef enum scir_hip_status {
+    SCIR_HIP_OK = 0,
+    SCIR_HIP_ERR_INVALID_ARGUMENT = 1, /* shapes/strides/pointers (GpuError::ShapeMismatch) */
+    SCIR_HIP_ERR_NO_DEVICE = 2,        /* no gfx950 HIP device (GpuError::BackendUnavailable) */
+    SCIR_HIP_ERR_BACKEND = 3,          /* a HIP runtime call failed; message names it */
+    SCIR_HIP_ERR_OUT_OF_MEMORY = 4,    /* device or pinned-host allocation failed */
+    SCIR_HIP_ERR_UNSUPPORTED = 5,      /* valid request this build does not implement */
+    SCIR_HIP_ERR_CAPTURE = 6           /* the call would allocate during stream capture */
+} scir_hip_status;
+
+/* ---- arithmetic selection --------------------------------------------- */
+typedef enum scir_hip_precision {
+    /* FP32 direct form, fused multiply-add, any summation order; error within
+     * (K+1) * 2^-23 * sum_k |b_k||x_{n-k}| of the exact sum. */
+    SCIR_HIP_PREC_F32 = 0,
+    /* FP32 direct form with a separately rounded multiply and add, taps
+     * accumulated in the reference's order (k = 0, 1, ..., K-1 starting from
+     * +0.0f): bit-identical to crates/scir-gpu/src/lib.rs:1134-1152. */
+    SCIR_HIP_PREC_F32_EXACT = 1,
+    /* Toeplitz-block contraction on MFMA: x and b split into three bf16 terms
+     * (hi+mid+lo), six cross products kept, FP32 accumulation. */
+    SCIR_HIP_PREC_BF16X3 = 2,
+    /* Toeplitz-block contraction on MFMA with one fp16 product (x and b
+     * rounded to fp16), FP32 accumulation.  Reported, not accuracy-bound. */
+    SCIR_HIP_PREC_FP16 = 3,
+    /* Fastest of F32 / BF16X3 for the shape (autotuned table when loaded,
+     * else built-in cut-over); never FP16. */
+    SCIR_HIP_PREC_AUTO = 4
+} scir_hip_precision;
+
+/* ---- runtime ------------------------------------------------------------ */
+/* Version of this ABI (major*10000 + minor*100 + patch). */
+SCIR_HIP_API int scir_hip_version(void);
+/* Initialise the runtime on the current HIP device (idempotent).  Replaces the
+ * per-call CudaCtx::create_default + cuModuleLoadData of
+ * crates/scir-gpu/src/lib.rs:601-622, 814-824: state is created once and kept. */
+SCIR_HIP_API int scir_hip_init(void);
+/* Release library-owned buffers on every device (idempotent). */
+SCIR_HIP_API int scir_hip_shutdown(void);
+/* Number of visible HIP devices (0 when none), like cuda_available
+ * (crates/scir-gpu/src/lib.rs:591-599); never fails for a missing device. */
+SCIR_HIP_API int scir_hip_device_count(int *count);
+/* Message of this thread's last failing call ("" if none).  Valid until the
+ * thread's next call into the library. */
+SCIR_HIP_API const char *scir_hip_last_error(void);
+
+/* ---- dense batched FIR ---------------------------------------------------- */
+/* Host tier: drop-in for cuda::fir1d_batched_f32_cuda
+ * (crates/scir-gpu/src/lib.rs:1036-1113).  x[B, ldx] and y[B, ldy] in host
+ * memory, b[K] host; copies in, filters, copies out, synchronises.  B or N == 0
+ * is a no-op; K == 0 writes zeros (the CPU loop's empty sum). */
+SCIR_HIP_API int scir_hip_fir_f32_host(const float *x, int64_t ldx, const float *b, int32_t K,
+                          int64_t B, int64_t N, float *y, int64_t ldy,
+                          int32_t precision);
+
+/* Device tier of the same operation (PTX kernel fir1d_batched_f32,
+ * crates/scir-gpu/src/lib.rs:727-811, and its launch at :1083-1099), for
+ * device-resident callers (torch tensors, benchmarks, graph capture). */
+SCIR_HIP_API int scir_hip_fir_f32_dev(const float *x, int64_t ldx, const float *b, int32_t K,
+                         int64_t B, int64_t N, float *y, int64_t ldy,
+                         int32_t precision, void *stream);
+
+/* CPU arm of the dispatcher: the reference's CPU fir1d_batched_f32
+ * (crates/scir-gpu/src/lib.rs:1134-1152) in the same arithmetic order (so the
+ * result is bit-identical), with rows spread over `nthreads` host threads
+ * (<= 0: all hardware threads).  Needs no device. */
+SCIR_HIP_API int scir_hip_fir_f32_cpu(const float *x, int64_t ldx, const float *b, int32_t K,
+                         int64_t B, int64_t N, float *y, int64_t ldy,
+                         int32_t nthreads);
+
+/* ---- ragged (CSR) batches -------------------------------------------------
+ * No counterpart in the reference (its FIR takes one dense Array2).  Signal s
+ * is x[offs[s] .. offs[s+1]) with taps b[b_offs[s] .. b_offs[s+1]); y has the
+ * same layout as x.  A plan holds the host-built, cost-sorted work list and
+ * the device work queue; executing it is one persistent launch.  Executing a
+ * plan does not allocate, so it may be captured in a HIP graph. */
+typedef struct scir_hip_ragged_plan_s *scir_hip_ragged_plan;
+/* offs[S+1], b_offs[S+1]: HOST arrays (int64, non-decreasing, offs[0] ==
+ * b_offs[0] == 0). */
+SCIR_HIP_API int scir_hip_ragged_plan_create(const int64_t *offs, const int64_t *b_offs, int64_t S,
+                                int32_t precision, scir_hip_ragged_plan *plan);
+SCIR_HIP_API int scir_hip_ragged_plan_execute(scir_hip_ragged_plan plan, const float *x,
+                                 const float *b, float *y, void *stream);
+SCIR_HIP_API int scir_hip_ragged_plan_destroy(scir_hip_ragged_plan plan);
+/* One-shot convenience: create + execute + destroy (synchronises the stream
+ * before freeing).  offs/b_offs are HOST arrays; x, b, y are device pointers. */
+SCIR_HIP_API int scir_hip_fir_ragged_f32_dev(const float *x, const int64_t *offs, const float *b,
+                                const int64_t *b_offs, int64_t S, float *y,
+                                int32_t precision, void *stream);
+/* Launch one plan per signal instead (S launches of the dense kernel) -- the
+ * baseline a persistent launch is measured against. */
+SCIR_HIP_API int scir_hip_ragged_plan_execute_per_signal(scir_hip_ragged_plan plan, const float *x,
+                                            const float *b, float *y, void *stream);
+
+/* ---- adjoints (vector-Jacobian products) ----------------------------------
+ * No counterpart in the reference.  For L = sum_{r,n} g[r,n] * y[r,n] with
+ * y = lfilter(b,[1],x) per row:
+ *   dx[r,m] = sum_k b[k] * g[r,m+k]           (terms with m+k < N)
+ *   db[k]   = sum_{r,n} g[r,n] * x[r,n-k]     (terms with n-k >= 0)
+ * In the reference's taps order the tap gradient is db reversed. */
+SCIR_HIP_API int scir_hip_fir_vjp_dx_dev(const float *g, int64_t ldg, const float *b, int32_t K,
+                            int64_t B, int64_t N, float *dx, int64_t lddx,
+                            int32_t precision, void *stream);
+/* db[K] is overwritten (device pointer).  Two-stage deterministic reduction:
+ * per-tile FP32 partial sums, then a fixed-order compensated sum. */
+SCIR_HIP_API int scir_hip_fir_vjp_db_dev(const float *g, int64_t ldg, const float *x, int64_t ldx,
+                            int32_t K, int64_t B, int64_t N, float *db, void *stream);
+
+/* ---- HIP graphs ------------------------------------------------------------
+ * Capture any sequence of *_dev / ragged-plan calls enqueued on `stream`
+ * between begin and end, then replay it.  Calls that would need to grow
+ * library scratch during capture fail with SCIR_HIP_ERR_CAPTURE: run the same
+ * call once eagerly first. */
+typedef struct scir_hip_graph_s *scir_hip_graph;
+SCIR_HIP_API int scir_hip_graph_begin(void *stream);
+SCIR_HIP_API int scir_hip_graph_end(void *stream, scir_hip_graph *graph);
+SCIR_HIP_API int scir_hip_graph_launch(scir_hip_graph graph, void *stream);
+SCIR_HIP_API int scir_hip_graph_destroy(scir_hip_graph graph);
+
+/* ---- autotuner -------------------------------------------------------------
+ * Grid-search tile / wavefront shapes and the F32-vs-MFMA cut-over on the
+ * current device for a set of (K, N) buckets, timing each candidate with HIP
+ * events, and write the winners as JSON to cache_path (NULL: keep in memory
+ * only).  budget_ms bounds the total search time (<= 0: default). */
+SCIR_HIP_API int scir_hip_autotune(const char *cache_path, int32_t budget_ms);
+/* Load a table written by scir_hip_autotune (entries for another arch are
+ * ignored). */
+SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
+
+/* ---- synthetic inputs -------------------------------------------------------
+ * Fill x[B, ldx] (device) with N(0,1) float32 samples from a counter-based
+ * generator (row r uses the stream seed ^ (r * 0x9E3779B97F4A7C15)); a
+ * fraction `outlier_frac` of samples, chosen by an independent hash bit
+ * stream, is multiplied by `outlier_scale`. */
+SCIR_HIP_API int scir_hip_synth_normal_f32(float *x, int64_t ldx, int64_t B, int64_t N, uint64_t seed,
+                              float outlier_frac, float outlier_scale, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCIR_HIP_H */

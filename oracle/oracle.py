@@ -1,0 +1,156 @@
+"""CPU oracle for the batched-FIR path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline
+leg import this module, and only as the checker; the product (``scir_amd``)
+never imports it.  It wraps ``oracle/libfir_oracle.so`` (built from
+``oracle/fir_oracle.c`` by ``oracle/Makefile``) with numpy-facing helpers.
+
+Functions and the reference lines they restate:
+
+* :func:`fir_ref_f32`  -- crates/scir-gpu/src/lib.rs:1134-1152, bit-faithful.
+* :func:`fir_ref_f64`  -- crates/scir-gpu/src/lib.rs:1166-1184.
+* :func:`lfilter_f64`  -- ``y = lfilter(b, [1], x)`` in FP64 on float32 inputs
+  (the convention the reference's function equals with ``b = taps[::-1]``).
+* :func:`abs_bound`    -- ``sum_k |b_k| |x_{n-k}|`` for the error bound.
+* :func:`vjp_f64`      -- FP64 reference gradient of ``sum(g * y)``.
+* :func:`lfilter_ragged_f64` -- the CSR batch form of :func:`lfilter_f64`.
+
+Every function is pinned by tests/test_oracle.py (see the C file header).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libfir_oracle.so")
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_i64 = ctypes.c_int64
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle library with oracle/Makefile (gcc, no FP contraction)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"],
+                       check=True)
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        lib = ctypes.CDLL(_LIB_PATH)
+        lib.oracle_fir_ref_f32.argtypes = [_f32p, _i64, _f32p, _i64, _i64, _i64, _f32p, _i64]
+        lib.oracle_fir_ref_f64.argtypes = [_f64p, _i64, _f64p, _i64, _i64, _i64, _f64p, _i64]
+        lib.oracle_lfilter_f64.argtypes = [_f32p, _i64, _f32p, _i64, _i64, _i64, _f64p, _i64]
+        lib.oracle_fir_absbound.argtypes = [_f32p, _i64, _f32p, _i64, _i64, _i64, _f64p, _i64]
+        lib.oracle_fir_vjp_f64.argtypes = [_f32p, _i64, _f32p, _i64, _f32p, _i64, _i64, _i64,
+                                           _f64p, _i64, _f64p]
+        lib.oracle_lfilter_ragged_f64.argtypes = [_f32p, _i64p, _f32p, _i64p, _i64, _f64p]
+        for fn in (lib.oracle_fir_ref_f32, lib.oracle_fir_ref_f64, lib.oracle_lfilter_f64,
+                   lib.oracle_fir_absbound, lib.oracle_fir_vjp_f64,
+                   lib.oracle_lfilter_ragged_f64):
+            fn.restype = None
+        _lib = lib
+    return _lib
+
+
+def _as2d(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    if a.ndim == 1:
+        a = a[None, :]
+    if a.ndim != 2:
+        raise ValueError("expected a 1-D or 2-D array")
+    return a
+
+
+def fir_ref_f32(x, taps) -> np.ndarray:
+    """Reference-order FIR in float32, bit-identical to the reference loop."""
+    x = _as2d(x, np.float32)
+    taps = np.ascontiguousarray(taps, dtype=np.float32).ravel()
+    B, N = x.shape
+    y = np.zeros((B, N), np.float32)
+    _load().oracle_fir_ref_f32(x, N, taps, taps.size, B, N, y, N)
+    return y
+
+
+def fir_ref_f64(x, taps) -> np.ndarray:
+    """Reference-order FIR in float64 (crates/scir-gpu/src/lib.rs:1166-1184)."""
+    x = _as2d(x, np.float64)
+    taps = np.ascontiguousarray(taps, dtype=np.float64).ravel()
+    B, N = x.shape
+    y = np.zeros((B, N), np.float64)
+    _load().oracle_fir_ref_f64(x, N, taps, taps.size, B, N, y, N)
+    return y
+
+
+def lfilter_f64(x, b) -> np.ndarray:
+    """FP64 ``lfilter(b, [1], x)`` along the last axis, float32 inputs."""
+    x = _as2d(x, np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32).ravel()
+    B, N = x.shape
+    y = np.zeros((B, N), np.float64)
+    _load().oracle_lfilter_f64(x, N, b, b.size, B, N, y, N)
+    return y
+
+
+def abs_bound(x, b) -> np.ndarray:
+    """``sum_k |b_k| |x_{n-k}|`` per output (float64)."""
+    x = _as2d(x, np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32).ravel()
+    B, N = x.shape
+    s = np.zeros((B, N), np.float64)
+    _load().oracle_fir_absbound(x, N, b, b.size, B, N, s, N)
+    return s
+
+
+def fir_error_bound(x, b) -> np.ndarray:
+    """Per-output bound ``(K+1) * 2^-23 * sum_k |b_k||x_{n-k}|`` on any FP32
+    evaluation order of the K-term sum (first-order gamma_K, strict form)."""
+    K = int(np.asarray(b).size)
+    return (K + 1) * 2.0 ** -23 * abs_bound(x, b)
+
+
+def vjp_f64(g, x, b):
+    """FP64 reference gradient of ``L = sum(g * lfilter(b,[1],x))``.
+
+    Returns ``(dx float64 [B,N], db float64 [K])``."""
+    g = _as2d(g, np.float32)
+    x = _as2d(x, np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32).ravel()
+    if g.shape != x.shape:
+        raise ValueError("g and x must have the same shape")
+    B, N = x.shape
+    dx = np.zeros((B, N), np.float64)
+    db = np.zeros(b.size, np.float64)
+    _load().oracle_fir_vjp_f64(g, N, x, N, b, b.size, B, N, dx, N, db)
+    return dx, db
+
+
+def lfilter_ragged_f64(x, offs, b, boffs) -> np.ndarray:
+    """FP64 ragged lfilter: signal s is ``x[offs[s]:offs[s+1]]`` filtered by
+    ``b[boffs[s]:boffs[s+1]]``."""
+    x = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    b = np.ascontiguousarray(b, dtype=np.float32).ravel()
+    offs = np.ascontiguousarray(offs, dtype=np.int64)
+    boffs = np.ascontiguousarray(boffs, dtype=np.int64)
+    y = np.zeros(x.size, np.float64)
+    _load().oracle_lfilter_ragged_f64(x, offs, b, boffs, offs.size - 1, y)
+    return y
+
+
+def rel_l2(y, y64) -> float:
+    """``||y - y64||_2 / ||y64||_2`` computed in float64."""
+    y = np.asarray(y, np.float64)
+    y64 = np.asarray(y64, np.float64)
+    den = np.linalg.norm(y64.ravel())
+    return float(np.linalg.norm((y - y64).ravel()) / (den if den > 0 else 1.0))

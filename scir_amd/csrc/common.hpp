@@ -1,0 +1,101 @@
+// Shared internals of libscir_hip.so: error reporting, per-device state and
+// library-owned scratch buffers.  Nothing here is part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "../../include/scir_hip.h"
+
+namespace scir {
+
+// Records `code` and a printf-style message as this thread's last error and
+// returns `code`, so call sites read `return fail(...)`.
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+#define SCIR_HIP_TRY(call)                                                            \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return ::scir::fail(e_ == hipErrorOutOfMemory ? SCIR_HIP_ERR_OUT_OF_MEMORY \
+                                                          : SCIR_HIP_ERR_BACKEND,     \
+                                "%s failed: %s", #call, hipGetErrorString(e_));       \
+    } while (0)
+
+#define SCIR_TRY(expr)              \
+    do {                            \
+        int s_ = (expr);            \
+        if (s_ != SCIR_HIP_OK)      \
+            return s_;              \
+    } while (0)
+
+// A grow-only device allocation owned by the library.
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+// Scratch that one stream's calls share.  Calls enqueued on the same stream
+// are ordered by the stream, so they may reuse the buffers; calls on other
+// streams get their own set.
+struct StreamScratch {
+    DeviceBuffer tap_frags;   // MFMA A-operand fragments of the current taps
+    DeviceBuffer partials;    // per-tile dL/db partial sums
+};
+
+struct DeviceState {
+    int device = -1;
+    bool ready = false;
+    int cu_count = 0;
+    int xcd_count = 8;
+    size_t lds_per_block = 0;
+    std::string arch;
+    std::mutex mu;  // guards everything below
+    std::map<void *, StreamScratch> scratch;
+    // Host tier: device copies of the caller's host arrays.
+    DeviceBuffer host_x, host_y, host_b;
+    hipStream_t host_stream = nullptr;
+};
+
+// Initialises (once) and returns the state of the current device.
+int current_device_state(DeviceState **out);
+
+// Makes `buf` hold at least `bytes`.  Growing frees the old block; a growth
+// requested while `stream` is capturing fails with SCIR_HIP_ERR_CAPTURE.
+int ensure_buffer(DeviceBuffer &buf, size_t bytes, hipStream_t stream);
+
+// Scratch of `stream` on device state `st` (caller holds st->mu).
+StreamScratch &scratch_for(DeviceState *st, hipStream_t stream);
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Validates the dense-call arguments shared by the FIR entry points.
+int check_dense_args(const void *x, int64_t ldx, const void *b, int32_t K, int64_t B,
+                     int64_t N, const void *y, int64_t ldy);
+
+// Launch-status check after a kernel launch.
+int check_launch(const char *what);
+
+// ---- dense entry points shared across translation units -----------------
+int fir_direct_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
+                   float *y, int64_t ldy, bool exact, hipStream_t stream, DeviceState *st);
+int fir_mfma_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
+                 float *y, int64_t ldy, int precision, hipStream_t stream, DeviceState *st);
+int fill_zero_2d(float *y, int64_t ldy, int64_t B, int64_t N, hipStream_t stream);
+
+// Autotuned choices (defaults until a table is loaded or a search ran).
+struct TuneChoice {
+    int direct_r = 16;        // outputs per lane in the direct kernel
+    int mfma_tiles = 2;       // 32x32 output tiles per wave in the MFMA kernel
+    int auto_mfma_min_k = 48; // AUTO uses BF16x3 MFMA from this K up
+};
+TuneChoice tune_lookup(int K, int64_t N);
+
+}  // namespace scir

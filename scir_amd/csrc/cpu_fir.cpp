@@ -1,0 +1,84 @@
+// CPU arm of the dispatcher (Device::Cpu): the reference's CPU FIR
+// (crates/scir-gpu/src/lib.rs:1134-1152) in the same arithmetic order, spread
+// over host threads by row.
+//
+// Each output is acc = 0.0f; acc = acc + (taps[K-1-t] * x[i-t]) for t = 0..,
+// i.e. acc = acc + b[t]*x[i-t] with b the lfilter-order taps, with the
+// multiply and the add rounded separately (this file is compiled with
+// -ffp-contract=off).  Blocks of 8 consecutive outputs are computed together
+// so the compiler can vectorise across outputs; each output keeps its own
+// sequential order, so the result is bit-identical to the reference loop.
+#include <algorithm>
+#include <cstdint>
+#include <thread>
+#include <vector>
+
+#include "../../include/scir_hip.h"
+
+namespace scir {
+int fail(int code, const char *fmt, ...);
+void clear_error();
+}  // namespace scir
+
+namespace {
+
+void fir_row(const float *xr, const float *b, int64_t K, int64_t N, float *yr)
+{
+    // Ramp-in: outputs whose history is shorter than K taps.
+    const int64_t ramp = std::min<int64_t>(N, K - 1);
+    for (int64_t i = 0; i < ramp; ++i) {
+        float acc = 0.0f;
+        for (int64_t t = 0; t <= i; ++t) acc = acc + b[t] * xr[i - t];
+        yr[i] = acc;
+    }
+    constexpr int V = 8;
+    int64_t i = ramp;
+    for (; i + V <= N; i += V) {
+        float acc[V] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int64_t t = 0; t < K; ++t) {
+            const float bt = b[t];
+            const float *xp = xr + i - t;
+            for (int j = 0; j < V; ++j) acc[j] = acc[j] + bt * xp[j];
+        }
+        for (int j = 0; j < V; ++j) yr[i + j] = acc[j];
+    }
+    for (; i < N; ++i) {
+        float acc = 0.0f;
+        for (int64_t t = 0; t < K; ++t) acc = acc + b[t] * xr[i - t];
+        yr[i] = acc;
+    }
+}
+
+}  // namespace
+
+extern "C" int scir_hip_fir_f32_cpu(const float *x, int64_t ldx, const float *b, int32_t K,
+                                    int64_t B, int64_t N, float *y, int64_t ldy,
+                                    int32_t nthreads)
+{
+    scir::clear_error();
+    if (B < 0 || N < 0 || K < 0)
+        return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "negative size");
+    if (B == 0 || N == 0) return SCIR_HIP_OK;
+    if (!x || !y || (K > 0 && !b))
+        return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "null pointer argument");
+    if (ldx < N || ldy < N)
+        return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row stride smaller than the row length");
+    if (K == 0) {
+        for (int64_t r = 0; r < B; ++r) std::fill(y + r * ldy, y + r * ldy + N, 0.0f);
+        return SCIR_HIP_OK;
+    }
+    int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
+    nt = (int)std::min<int64_t>(nt, B);
+    auto work = [&](int w) {
+        for (int64_t r = w; r < B; r += nt) fir_row(x + r * ldx, b, K, N, y + r * ldy);
+    };
+    if (nt <= 1) {
+        work(0);
+        return SCIR_HIP_OK;
+    }
+    std::vector<std::thread> pool;
+    pool.reserve(nt);
+    for (int w = 0; w < nt; ++w) pool.emplace_back(work, w);
+    for (auto &t : pool) t.join();
+    return SCIR_HIP_OK;
+}

@@ -1,0 +1,297 @@
+// Direct-form FP32 batched FIR on gfx950, and the dense entry points.
+//
+// Replaces the reference's one-thread-per-output PTX kernel
+// (crates/scir-gpu/src/lib.rs:727-811: two global loads per tap, integer
+// div/rem per thread, no reuse) with a tiled design:
+//
+//   * one workgroup (256 lanes, 4 waves) = one tile of T = 256*R consecutive
+//     outputs of one row;
+//   * the tile's inputs plus the (K-1)-sample history are staged once into
+//     LDS with 16-byte loads (zero-filled before the row start and past its
+//     end: zero initial state, output length == input length);
+//   * each lane owns R consecutive outputs and slides a register window over
+//     the LDS copy, so each staged sample is read from LDS ~(R+32)/R times
+//     and each tap feeds R multiply-adds from one scalar register;
+//   * taps are read with uniform (scalar) loads and applied 32 at a time;
+//     remainders use chunks of 16, 8, 4, 2, 1 taps, so no tap is padded and
+//     non-finite inputs behave as in the reference loop;
+//   * long filters are processed in super-chunks of 2048 taps so the LDS
+//     footprint stays bounded for any K.
+//
+// LDS layout: sample s lives at s + 4*(s/16) (4 pad words after every 16),
+// which makes the 16-float lane stride of the window reads bank-conflict free.
+//
+// Two arithmetic modes (template EXACT):
+//   EXACT = false: fused multiply-add in any order -- error within
+//                  (K+1)*2^-23*sum|b||x| of the exact sum;
+//   EXACT = true : separately rounded multiply and add, taps accumulated in
+//                  the order k = 0..K-1 from +0.0f, which is the reference
+//                  CPU loop's order (crates/scir-gpu/src/lib.rs:1141-1148:
+//                  taps[K-1-t]*x[i-t] for t ascending == b[t]*x[i-t]).  Extra
+//                  terms past the row start add +-0 to the sum, which leaves
+//                  every bit unchanged (the sum never becomes -0.0), so the
+//                  output is bit-identical to the reference.
+#include "common.hpp"
+
+#include <algorithm>
+
+namespace scir {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kSuper = 2048;  // taps per LDS super-chunk (multiple of 32)
+constexpr int kChunk = 32;    // taps per fully unrolled step
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int pad16(int s) { return s + ((s >> 4) << 2); }
+
+template <bool EXACT>
+__device__ __forceinline__ float mac(float acc, float t, float xv);
+
+template <>
+__device__ __forceinline__ float mac<true>(float acc, float t, float xv)
+{
+#pragma clang fp contract(off)
+    float p = t * xv;
+    return acc + p;
+}
+
+template <>
+__device__ __forceinline__ float mac<false>(float acc, float t, float xv)
+{
+    return __builtin_fmaf(t, xv, acc);
+}
+
+// acc[r] += sum_{kk < C} bk[kk] * x[i_r - k0 - kk], where the LDS window
+// starts OFF floats below the 4-aligned unpadded index `wa`:
+//   x[i_r - k0 - kk] == xs[wa + OFF + r + C - kk].
+template <int R, int C, int OFF, bool EXACT>
+__device__ __forceinline__ void taps_chunk(float (&acc)[R], const float *xs, int wa,
+                                           const float *__restrict__ bk)
+{
+    constexpr int W4 = (R + C + OFF + 3) / 4;
+    float w[W4 * 4];
+#pragma unroll
+    for (int j = 0; j < W4; ++j) {
+        f32x4 v = *reinterpret_cast<const f32x4 *>(xs + pad16(wa + 4 * j));
+        w[4 * j + 0] = v.x;
+        w[4 * j + 1] = v.y;
+        w[4 * j + 2] = v.z;
+        w[4 * j + 3] = v.w;
+    }
+#pragma unroll
+    for (int kk = 0; kk < C; ++kk) {
+        const float t = bk[kk];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = mac<EXACT>(acc[r], t, w[OFF + r + C - kk]);
+    }
+}
+
+// Stages x[row, g0 .. g0 + count) (count a multiple of 4, g0 a multiple of 4)
+// into xs (padded layout); samples outside [0, N) read as zero.
+__device__ __forceinline__ void stage_row(float *xs, const float *__restrict__ xrow, int64_t g0,
+                                          int count, int64_t N, bool vec_ok)
+{
+    for (int v = threadIdx.x; v < count / 4; v += kThreads) {
+        int64_t g = g0 + 4 * (int64_t)v;
+        f32x4 val;
+        if (vec_ok && g >= 0 && g + 3 < N) {
+            val = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g));
+        } else {
+            val.x = (g + 0 >= 0 && g + 0 < N) ? xrow[g + 0] : 0.0f;
+            val.y = (g + 1 >= 0 && g + 1 < N) ? xrow[g + 1] : 0.0f;
+            val.z = (g + 2 >= 0 && g + 2 < N) ? xrow[g + 2] : 0.0f;
+            val.w = (g + 3 >= 0 && g + 3 < N) ? xrow[g + 3] : 0.0f;
+        }
+        *reinterpret_cast<f32x4 *>(xs + pad16(4 * v)) = val;
+    }
+}
+
+template <int R, bool EXACT>
+__global__ __launch_bounds__(kThreads) void fir_direct_kernel(
+    const float *__restrict__ x, int64_t ldx, const float *__restrict__ b, int K, int64_t N,
+    float *__restrict__ y, int64_t ldy, int64_t row0, int tiles_per_row, int vec_x, int vec_y)
+{
+    extern __shared__ float xs[];
+    constexpr int T = kThreads * R;
+    const int64_t row = row0 + blockIdx.x / tiles_per_row;
+    const int64_t t0 = (int64_t)(blockIdx.x % tiles_per_row) * T;
+    const float *__restrict__ xrow = x + row * ldx;
+    const int lane_base = threadIdx.x * R;
+
+    float acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0f;
+
+    for (int ks = 0; ks < K; ks += kSuper) {
+        const int kcount = min(kSuper, K - ks);
+        const int H = (kcount + 6) & ~3;  // >= kcount + 3, multiple of 4
+        if (ks > 0) __syncthreads();      // previous super-chunk done with xs
+        stage_row(xs, xrow, t0 - ks - H, T + H + 4, N, vec_x != 0);
+        __syncthreads();
+
+        const float *__restrict__ bs = b + ks;
+        int kc = 0;
+        for (; kc + kChunk <= kcount; kc += kChunk)
+            taps_chunk<R, kChunk, 0, EXACT>(acc, xs, H + lane_base - kc - kChunk, bs + kc);
+        const int rem = kcount - kc;
+        if (rem & 16) {
+            taps_chunk<R, 16, 0, EXACT>(acc, xs, H + lane_base - kc - 16, bs + kc);
+            kc += 16;
+        }
+        if (rem & 8) {
+            taps_chunk<R, 8, 0, EXACT>(acc, xs, H + lane_base - kc - 8, bs + kc);
+            kc += 8;
+        }
+        if (rem & 4) {
+            taps_chunk<R, 4, 0, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
+            kc += 4;
+        }
+        if (rem & 2) {  // kc % 4 == 0 here; window starts 2 below a 4-aligned index
+            taps_chunk<R, 2, 2, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
+            kc += 2;
+        }
+        if (rem & 1) {
+            if (kc & 2)  // kc % 4 == 2: window starts 1 below the aligned index
+                taps_chunk<R, 1, 1, EXACT>(acc, xs, H + lane_base - kc - 2, bs + kc);
+            else         // kc % 4 == 0: window starts 3 below the aligned index
+                taps_chunk<R, 1, 3, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
+        }
+    }
+
+    float *__restrict__ yrow = y + row * ldy;
+    const int64_t i0 = t0 + lane_base;
+    if (vec_y && i0 + R <= N) {
+#pragma unroll
+        for (int r = 0; r < R; r += 4) {
+            f32x4 v = {acc[r], acc[r + 1], acc[r + 2], acc[r + 3]};
+            __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(yrow + i0 + r));
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (i0 + r < N) yrow[i0 + r] = acc[r];
+    }
+}
+
+template <int R, bool EXACT>
+int launch_direct(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
+                  float *y, int64_t ldy, hipStream_t stream)
+{
+    constexpr int T = kThreads * R;
+    const int kmax = std::min(K, kSuper);
+    const int H = (kmax + 6) & ~3;
+    const int staged = T + H + 4;
+    const size_t lds = (size_t)(staged + (staged / 16) * 4 + 4) * sizeof(float);
+    const int64_t tiles = (N + T - 1) / T;
+    if (tiles > 0x7fffffff)
+        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row too long (N=%lld)", (long long)N);
+    const int vec_x = ((reinterpret_cast<uintptr_t>(x) & 15) == 0) && (ldx % 4 == 0);
+    const int vec_y = ((reinterpret_cast<uintptr_t>(y) & 15) == 0) && (ldy % 4 == 0);
+    // One launch covers as many rows as fit in a 2^31-1 workgroup grid.
+    const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / tiles);
+    for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
+        const int64_t rows = std::min(rows_per_launch, B - r0);
+        hipLaunchKernelGGL((fir_direct_kernel<R, EXACT>), dim3((unsigned)(rows * tiles)),
+                           dim3(kThreads), lds, stream, x, ldx, b, K, N, y, ldy, r0, (int)tiles,
+                           vec_x, vec_y);
+        SCIR_TRY(check_launch("fir_direct_kernel"));
+    }
+    return SCIR_HIP_OK;
+}
+
+}  // namespace
+
+int fir_direct_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
+                   float *y, int64_t ldy, bool exact, hipStream_t stream, DeviceState *)
+{
+    if (B == 0 || N == 0) return SCIR_HIP_OK;
+    if (K == 0) return fill_zero_2d(y, ldy, B, N, stream);
+    const TuneChoice tc = tune_lookup(K, N);
+    const bool small = (tc.direct_r == 8) || N <= 2048;
+    if (exact)
+        return small ? launch_direct<8, true>(x, ldx, b, K, B, N, y, ldy, stream)
+                     : launch_direct<16, true>(x, ldx, b, K, B, N, y, ldy, stream);
+    return small ? launch_direct<8, false>(x, ldx, b, K, B, N, y, ldy, stream)
+                 : launch_direct<16, false>(x, ldx, b, K, B, N, y, ldy, stream);
+}
+
+// Dense dispatch shared by the host and device tiers.
+static int fir_dense_dispatch(const float *x, int64_t ldx, const float *b, int32_t K, int64_t B,
+                              int64_t N, float *y, int64_t ldy, int32_t precision,
+                              hipStream_t stream, DeviceState *st)
+{
+    switch (precision) {
+    case SCIR_HIP_PREC_F32:
+        return fir_direct_dev(x, ldx, b, K, B, N, y, ldy, false, stream, st);
+    case SCIR_HIP_PREC_F32_EXACT:
+        return fir_direct_dev(x, ldx, b, K, B, N, y, ldy, true, stream, st);
+    case SCIR_HIP_PREC_BF16X3:
+    case SCIR_HIP_PREC_FP16:
+        return fir_mfma_dev(x, ldx, b, K, B, N, y, ldy, precision, stream, st);
+    case SCIR_HIP_PREC_AUTO: {
+        const TuneChoice tc = tune_lookup(K, N);
+        if (K >= tc.auto_mfma_min_k)
+            return fir_mfma_dev(x, ldx, b, K, B, N, y, ldy, SCIR_HIP_PREC_BF16X3, stream, st);
+        return fir_direct_dev(x, ldx, b, K, B, N, y, ldy, false, stream, st);
+    }
+    default:
+        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "unknown precision %d", precision);
+    }
+}
+
+}  // namespace scir
+
+using namespace scir;
+
+extern "C" {
+
+int scir_hip_fir_f32_dev(const float *x, int64_t ldx, const float *b, int32_t K, int64_t B,
+                         int64_t N, float *y, int64_t ldy, int32_t precision, void *stream)
+{
+    clear_error();
+    SCIR_TRY(check_dense_args(x, ldx, b, K, B, N, y, ldy));
+    if (B == 0 || N == 0) return SCIR_HIP_OK;
+    DeviceState *st = nullptr;
+    SCIR_TRY(current_device_state(&st));
+    std::lock_guard<std::mutex> g(st->mu);
+    return fir_dense_dispatch(x, ldx, b, K, B, N, y, ldy, precision, as_stream(stream), st);
+}
+
+int scir_hip_fir_f32_host(const float *x, int64_t ldx, const float *b, int32_t K, int64_t B,
+                          int64_t N, float *y, int64_t ldy, int32_t precision)
+{
+    clear_error();
+    SCIR_TRY(check_dense_args(x, ldx, b, K, B, N, y, ldy));
+    if (B == 0 || N == 0) return SCIR_HIP_OK;
+    if (precision < SCIR_HIP_PREC_F32 || precision > SCIR_HIP_PREC_AUTO)
+        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "unknown precision %d", precision);
+    DeviceState *st = nullptr;
+    SCIR_TRY(current_device_state(&st));
+    std::lock_guard<std::mutex> g(st->mu);
+    if (!st->host_stream) SCIR_HIP_TRY(hipStreamCreateWithFlags(&st->host_stream, hipStreamNonBlocking));
+    hipStream_t s = st->host_stream;
+    const size_t row_bytes = (size_t)N * sizeof(float);
+    // Device rows are padded to a multiple of 4 floats so the kernels can use
+    // 16-byte accesses.
+    const int64_t ldd = (N + 3) & ~int64_t(3);
+    const size_t mat_bytes = (size_t)B * (size_t)ldd * sizeof(float);
+    SCIR_TRY(ensure_buffer(st->host_x, mat_bytes, s));
+    SCIR_TRY(ensure_buffer(st->host_y, mat_bytes, s));
+    SCIR_TRY(ensure_buffer(st->host_b, (size_t)std::max(K, 1) * sizeof(float), s));
+    float *dx = static_cast<float *>(st->host_x.ptr);
+    float *dy = static_cast<float *>(st->host_y.ptr);
+    float *db = static_cast<float *>(st->host_b.ptr);
+    SCIR_HIP_TRY(hipMemcpy2DAsync(dx, ldd * sizeof(float), x, ldx * sizeof(float), row_bytes, B,
+                                  hipMemcpyHostToDevice, s));
+    if (K > 0)
+        SCIR_HIP_TRY(hipMemcpyAsync(db, b, (size_t)K * sizeof(float), hipMemcpyHostToDevice, s));
+    SCIR_TRY(fir_dense_dispatch(dx, ldd, db, K, B, N, dy, ldd, precision, s, st));
+    SCIR_HIP_TRY(hipMemcpy2DAsync(y, ldy * sizeof(float), dy, ldd * sizeof(float), row_bytes, B,
+                                  hipMemcpyDeviceToHost, s));
+    SCIR_HIP_TRY(hipStreamSynchronize(s));
+    return SCIR_HIP_OK;
+}
+
+}  // extern "C"
