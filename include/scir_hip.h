@@ -172,6 +172,14 @@ SCIR_HIP_API int scir_hip_autotune(const char *cache_path, int32_t budget_ms);
 /* Load a table written by scir_hip_autotune (entries for another arch are
  * ignored). */
 SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
+/* Force one tuning parameter for every later call in this process, overriding
+ * the table; value < 0 removes the override.  Keys:
+ *   "direct.r"        outputs per lane of the direct kernel (8 or 16)
+ *   "direct.loader"   0 = LDS-staged tile, 1 = per-lane global window
+ *   "mfma.tiles"      32x32 output tiles per wave of the MFMA kernel (1, 2, 4)
+ *   "auto.mfma_min_k" smallest K for which AUTO picks BF16x3 MFMA
+ * The autotuner's search uses the same mechanism. */
+SCIR_HIP_API int scir_hip_tune_set(const char *key, int32_t value);
 
 /* ---- synthetic inputs -------------------------------------------------------
  * Fill x[B, ldx] (device) with N(0,1) float32 samples from a counter-based

@@ -62,6 +62,7 @@ SIGNATURES = {
     "scir_hip_graph_destroy": (_I, [_P]),
     "scir_hip_autotune": (_I, [_c.c_char_p, _I32]),
     "scir_hip_autotune_load": (_I, [_c.c_char_p]),
+    "scir_hip_tune_set": (_I, [_c.c_char_p, _I32]),
     "scir_hip_synth_normal_f32": (_I, [_P, _I64, _I64, _I64, _U64, _F, _F, _P]),
 }
 

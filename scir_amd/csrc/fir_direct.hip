@@ -88,6 +88,39 @@ __device__ __forceinline__ void taps_chunk(float (&acc)[R], const float *xs, int
     }
 }
 
+// Same contraction with the window read straight from global memory:
+//   x[i_r - k0 - kk] == xrow[ga + OFF + r + C - kk],  ga 4-aligned.
+// GUARD = true reads element by element and substitutes 0 outside [0, N).
+template <int R, int C, int OFF, bool EXACT, bool GUARD>
+__device__ __forceinline__ void taps_chunk_global(float (&acc)[R], const float *__restrict__ xrow,
+                                                  int64_t ga, int64_t N,
+                                                  const float *__restrict__ bk)
+{
+    constexpr int W4 = (R + C + OFF + 3) / 4;
+    float w[W4 * 4];
+#pragma unroll
+    for (int j = 0; j < W4; ++j) {
+        const int64_t g = ga + 4 * j;
+        if (!GUARD) {
+            f32x4 v = *reinterpret_cast<const f32x4 *>(xrow + g);
+            w[4 * j + 0] = v.x;
+            w[4 * j + 1] = v.y;
+            w[4 * j + 2] = v.z;
+            w[4 * j + 3] = v.w;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                w[4 * j + c] = (g + c >= 0 && g + c < N) ? xrow[g + c] : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int kk = 0; kk < C; ++kk) {
+        const float t = bk[kk];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = mac<EXACT>(acc[r], t, w[OFF + r + C - kk]);
+    }
+}
+
 // Stages x[row, g0 .. g0 + count) (count a multiple of 4, g0 a multiple of 4)
 // into xs (padded layout); samples outside [0, N) read as zero.
 __device__ __forceinline__ void stage_row(float *xs, const float *__restrict__ xrow, int64_t g0,
@@ -107,6 +140,10 @@ __device__ __forceinline__ void stage_row(float *xs, const float *__restrict__ x
         *reinterpret_cast<f32x4 *>(xs + pad16(4 * v)) = val;
     }
 }
+
+template <int R>
+__device__ __forceinline__ void store_outputs(const float (&acc)[R], float *__restrict__ yrow,
+                                              int64_t i0, int64_t N, bool vec_y);
 
 template <int R, bool EXACT>
 __global__ __launch_bounds__(kThreads) void fir_direct_kernel(
@@ -160,8 +197,48 @@ __global__ __launch_bounds__(kThreads) void fir_direct_kernel(
         }
     }
 
-    float *__restrict__ yrow = y + row * ldy;
-    const int64_t i0 = t0 + lane_base;
+    store_outputs<R>(acc, y + row * ldy, t0 + lane_base, N, vec_y != 0);
+}
+
+// All K taps for R consecutive outputs starting at i0, window from global
+// memory.  Chunk order is k ascending (the EXACT mode relies on it).
+template <int R, bool EXACT, bool GUARD>
+__device__ __forceinline__ void taps_all_global(float (&acc)[R], const float *__restrict__ xrow,
+                                                int64_t i0, int K, int64_t N,
+                                                const float *__restrict__ b)
+{
+    int kc = 0;
+    for (; kc + kChunk <= K; kc += kChunk)
+        taps_chunk_global<R, kChunk, 0, EXACT, GUARD>(acc, xrow, i0 - kc - kChunk, N, b + kc);
+    const int rem = K - kc;
+    if (rem & 16) {
+        taps_chunk_global<R, 16, 0, EXACT, GUARD>(acc, xrow, i0 - kc - 16, N, b + kc);
+        kc += 16;
+    }
+    if (rem & 8) {
+        taps_chunk_global<R, 8, 0, EXACT, GUARD>(acc, xrow, i0 - kc - 8, N, b + kc);
+        kc += 8;
+    }
+    if (rem & 4) {
+        taps_chunk_global<R, 4, 0, EXACT, GUARD>(acc, xrow, i0 - kc - 4, N, b + kc);
+        kc += 4;
+    }
+    if (rem & 2) {
+        taps_chunk_global<R, 2, 2, EXACT, GUARD>(acc, xrow, i0 - kc - 4, N, b + kc);
+        kc += 2;
+    }
+    if (rem & 1) {
+        if (kc & 2)
+            taps_chunk_global<R, 1, 1, EXACT, GUARD>(acc, xrow, i0 - kc - 2, N, b + kc);
+        else
+            taps_chunk_global<R, 1, 3, EXACT, GUARD>(acc, xrow, i0 - kc - 4, N, b + kc);
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void store_outputs(const float (&acc)[R], float *__restrict__ yrow,
+                                              int64_t i0, int64_t N, bool vec_y)
+{
     if (vec_y && i0 + R <= N) {
 #pragma unroll
         for (int r = 0; r < R; r += 4) {
@@ -173,6 +250,52 @@ __global__ __launch_bounds__(kThreads) void fir_direct_kernel(
         for (int r = 0; r < R; ++r)
             if (i0 + r < N) yrow[i0 + r] = acc[r];
     }
+}
+
+// Variant without LDS or barriers: each lane loads its own (R+K-1)-sample
+// window from global memory; neighbouring lanes' overlapping windows are
+// served by the L1/L2 caches, so HBM still sees each sample about once.
+// Interior tiles use unguarded 16-byte loads.
+template <int R, bool EXACT>
+__global__ __launch_bounds__(kThreads) void fir_direct_global_kernel(
+    const float *__restrict__ x, int64_t ldx, const float *__restrict__ b, int K, int64_t N,
+    float *__restrict__ y, int64_t ldy, int64_t row0, int tiles_per_row, int vec_x, int vec_y)
+{
+    constexpr int T = kThreads * R;
+    const int64_t row = row0 + blockIdx.x / tiles_per_row;
+    const int64_t t0 = (int64_t)(blockIdx.x % tiles_per_row) * T;
+    const float *__restrict__ xrow = x + row * ldx;
+    const int64_t i0 = t0 + threadIdx.x * R;
+    float acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0f;
+    const bool interior = vec_x && t0 >= (int64_t)K + 3 && t0 + T + 3 <= N;
+    if (interior)
+        taps_all_global<R, EXACT, false>(acc, xrow, i0, K, N, b);
+    else
+        taps_all_global<R, EXACT, true>(acc, xrow, i0, K, N, b);
+    store_outputs<R>(acc, y + row * ldy, i0, N, vec_y != 0);
+}
+
+template <int R, bool EXACT>
+int launch_direct_global(const float *x, int64_t ldx, const float *b, int K, int64_t B,
+                         int64_t N, float *y, int64_t ldy, hipStream_t stream)
+{
+    constexpr int T = kThreads * R;
+    const int64_t tiles = (N + T - 1) / T;
+    if (tiles > 0x7fffffff)
+        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row too long (N=%lld)", (long long)N);
+    const int vec_x = ((reinterpret_cast<uintptr_t>(x) & 15) == 0) && (ldx % 4 == 0);
+    const int vec_y = ((reinterpret_cast<uintptr_t>(y) & 15) == 0) && (ldy % 4 == 0);
+    const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / tiles);
+    for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
+        const int64_t rows = std::min(rows_per_launch, B - r0);
+        hipLaunchKernelGGL((fir_direct_global_kernel<R, EXACT>), dim3((unsigned)(rows * tiles)),
+                           dim3(kThreads), 0, stream, x, ldx, b, K, N, y, ldy, r0, (int)tiles,
+                           vec_x, vec_y);
+        SCIR_TRY(check_launch("fir_direct_global_kernel"));
+    }
+    return SCIR_HIP_OK;
 }
 
 template <int R, bool EXACT>
@@ -209,12 +332,19 @@ int fir_direct_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B
     if (B == 0 || N == 0) return SCIR_HIP_OK;
     if (K == 0) return fill_zero_2d(y, ldy, B, N, stream);
     const TuneChoice tc = tune_lookup(K, N);
-    const bool small = (tc.direct_r == 8) || N <= 2048;
+    const bool r8 = (tc.direct_r == 8) || N <= 2048;
+    if (tc.direct_loader == 1) {
+        if (exact)
+            return r8 ? launch_direct_global<8, true>(x, ldx, b, K, B, N, y, ldy, stream)
+                      : launch_direct_global<16, true>(x, ldx, b, K, B, N, y, ldy, stream);
+        return r8 ? launch_direct_global<8, false>(x, ldx, b, K, B, N, y, ldy, stream)
+                  : launch_direct_global<16, false>(x, ldx, b, K, B, N, y, ldy, stream);
+    }
     if (exact)
-        return small ? launch_direct<8, true>(x, ldx, b, K, B, N, y, ldy, stream)
-                     : launch_direct<16, true>(x, ldx, b, K, B, N, y, ldy, stream);
-    return small ? launch_direct<8, false>(x, ldx, b, K, B, N, y, ldy, stream)
-                 : launch_direct<16, false>(x, ldx, b, K, B, N, y, ldy, stream);
+        return r8 ? launch_direct<8, true>(x, ldx, b, K, B, N, y, ldy, stream)
+                  : launch_direct<16, true>(x, ldx, b, K, B, N, y, ldy, stream);
+    return r8 ? launch_direct<8, false>(x, ldx, b, K, B, N, y, ldy, stream)
+              : launch_direct<16, false>(x, ldx, b, K, B, N, y, ldy, stream);
 }
 
 // Dense dispatch shared by the host and device tiers.

@@ -32,6 +32,18 @@ def host_fir(hip, x, b, precision):
     return y
 
 
+@pytest.fixture(params=[(0, 16), (0, 8), (1, 16), (1, 8)], ids=lambda p: f"loader{p[0]}-r{p[1]}")
+def variant(request, hip):
+    """Each direct-kernel variant (LDS-staged / global window, 8 or 16 outputs per lane)."""
+    loader, r = request.param
+    L = hip.lib()
+    hip.check(L.scir_hip_tune_set(b"direct.loader", loader))
+    hip.check(L.scir_hip_tune_set(b"direct.r", r))
+    yield request.param
+    hip.check(L.scir_hip_tune_set(b"direct.loader", -1))
+    hip.check(L.scir_hip_tune_set(b"direct.r", -1))
+
+
 def test_reference_kat(hip):
     x = np.array([[1.0, 2.0, 3.0, 4.0], [0.5, 0.0, -0.5, -1.0]], np.float32)
     taps = np.array([0.25, 0.5, 0.25], np.float32)
@@ -46,7 +58,7 @@ N_CASES = [1, 3, 100, 4095, 4096, 4097, 9001]
 
 
 @pytest.mark.parametrize("K", K_CASES)
-def test_exact_bitwise_vs_reference_loop(hip, oracle, K):
+def test_exact_bitwise_vs_reference_loop(hip, oracle, variant, K):
     r = rng(100 + K)
     for N in N_CASES:
         B = 3 if N < 5000 else 2
@@ -59,7 +71,7 @@ def test_exact_bitwise_vs_reference_loop(hip, oracle, K):
 
 
 @pytest.mark.parametrize("K", [1, 5, 33, 129, 255, 1000, 2048, 3001])
-def test_f32_within_bound_of_fp64(hip, oracle, K):
+def test_f32_within_bound_of_fp64(hip, oracle, variant, K):
     r = rng(200 + K)
     for N in (17, 4096, 10007):
         x = r.standard_normal((3, N)).astype(np.float32)
@@ -82,7 +94,7 @@ def test_k0_zero_and_empty_shapes(hip):
     assert L.scir_hip_fir_f32_host(p, 50, p, 3, 3, 0, p, 50, 0) == hip.OK
 
 
-def test_device_tier_strided_and_unaligned(hip, oracle):
+def test_device_tier_strided_and_unaligned(hip, oracle, variant):
     import torch
     from scir_amd import device
     r = rng(3)
