@@ -175,7 +175,8 @@ SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
 /* Force one tuning parameter for every later call in this process, overriding
  * the table; value < 0 removes the override.  Keys:
  *   "direct.r"        outputs per lane of the direct kernel (8 or 16)
- *   "direct.loader"   0 = LDS-staged tile, 1 = per-lane global window
+ *   "direct.loader"   0 = one tile per workgroup, 1 = pipelined row segments
+ *   "direct.seg_tiles" tiles per workgroup of the pipelined direct kernel
  *   "mfma.tiles"      32x32 output tiles per wave of the MFMA kernel (1, 2, 4)
  *   "auto.mfma_min_k" smallest K for which AUTO picks BF16x3 MFMA
  * The autotuner's search uses the same mechanism. */

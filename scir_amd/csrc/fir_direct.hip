@@ -88,39 +88,6 @@ __device__ __forceinline__ void taps_chunk(float (&acc)[R], const float *xs, int
     }
 }
 
-// Same contraction with the window read straight from global memory:
-//   x[i_r - k0 - kk] == xrow[ga + OFF + r + C - kk],  ga 4-aligned.
-// GUARD = true reads element by element and substitutes 0 outside [0, N).
-template <int R, int C, int OFF, bool EXACT, bool GUARD>
-__device__ __forceinline__ void taps_chunk_global(float (&acc)[R], const float *__restrict__ xrow,
-                                                  int64_t ga, int64_t N,
-                                                  const float *__restrict__ bk)
-{
-    constexpr int W4 = (R + C + OFF + 3) / 4;
-    float w[W4 * 4];
-#pragma unroll
-    for (int j = 0; j < W4; ++j) {
-        const int64_t g = ga + 4 * j;
-        if (!GUARD) {
-            f32x4 v = *reinterpret_cast<const f32x4 *>(xrow + g);
-            w[4 * j + 0] = v.x;
-            w[4 * j + 1] = v.y;
-            w[4 * j + 2] = v.z;
-            w[4 * j + 3] = v.w;
-        } else {
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                w[4 * j + c] = (g + c >= 0 && g + c < N) ? xrow[g + c] : 0.0f;
-        }
-    }
-#pragma unroll
-    for (int kk = 0; kk < C; ++kk) {
-        const float t = bk[kk];
-#pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = mac<EXACT>(acc[r], t, w[OFF + r + C - kk]);
-    }
-}
-
 // Stages x[row, g0 .. g0 + count) (count a multiple of 4, g0 a multiple of 4)
 // into xs (padded layout); samples outside [0, N) read as zero.
 __device__ __forceinline__ void stage_row(float *xs, const float *__restrict__ xrow, int64_t g0,
@@ -141,10 +108,59 @@ __device__ __forceinline__ void stage_row(float *xs, const float *__restrict__ x
     }
 }
 
+// acc[r] += sum_{k' < kcount} bs[k'] * x[i_r - k'] with the window in LDS:
+// x[i_r - k'] == xs[H + lane_base + r - k'].  Chunks run in ascending k'.
+template <int R, bool EXACT>
+__device__ __forceinline__ void taps_from_lds(float (&acc)[R], const float *xs, int H,
+                                              int lane_base, int kcount,
+                                              const float *__restrict__ bs)
+{
+    int kc = 0;
+    for (; kc + kChunk <= kcount; kc += kChunk)
+        taps_chunk<R, kChunk, 0, EXACT>(acc, xs, H + lane_base - kc - kChunk, bs + kc);
+    const int rem = kcount - kc;
+    if (rem & 16) {
+        taps_chunk<R, 16, 0, EXACT>(acc, xs, H + lane_base - kc - 16, bs + kc);
+        kc += 16;
+    }
+    if (rem & 8) {
+        taps_chunk<R, 8, 0, EXACT>(acc, xs, H + lane_base - kc - 8, bs + kc);
+        kc += 8;
+    }
+    if (rem & 4) {
+        taps_chunk<R, 4, 0, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
+        kc += 4;
+    }
+    if (rem & 2) {  // kc % 4 == 0 here; window starts 2 below a 4-aligned index
+        taps_chunk<R, 2, 2, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
+        kc += 2;
+    }
+    if (rem & 1) {
+        if (kc & 2)  // kc % 4 == 2: window starts 1 below the aligned index
+            taps_chunk<R, 1, 1, EXACT>(acc, xs, H + lane_base - kc - 2, bs + kc);
+        else         // kc % 4 == 0: window starts 3 below the aligned index
+            taps_chunk<R, 1, 3, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
+    }
+}
+
 template <int R>
 __device__ __forceinline__ void store_outputs(const float (&acc)[R], float *__restrict__ yrow,
-                                              int64_t i0, int64_t N, bool vec_y);
+                                              int64_t i0, int64_t N, bool vec_y)
+{
+    if (vec_y && i0 + R <= N) {
+#pragma unroll
+        for (int r = 0; r < R; r += 4) {
+            f32x4 v = {acc[r], acc[r + 1], acc[r + 2], acc[r + 3]};
+            __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(yrow + i0 + r));
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (i0 + r < N) yrow[i0 + r] = acc[r];
+    }
+}
 
+// One workgroup = one tile; any K (taps in LDS super-chunks of kSuper).
 template <int R, bool EXACT>
 __global__ __launch_bounds__(kThreads) void fir_direct_kernel(
     const float *__restrict__ x, int64_t ldx, const float *__restrict__ b, int K, int64_t N,
@@ -167,135 +183,78 @@ __global__ __launch_bounds__(kThreads) void fir_direct_kernel(
         if (ks > 0) __syncthreads();      // previous super-chunk done with xs
         stage_row(xs, xrow, t0 - ks - H, T + H + 4, N, vec_x != 0);
         __syncthreads();
-
-        const float *__restrict__ bs = b + ks;
-        int kc = 0;
-        for (; kc + kChunk <= kcount; kc += kChunk)
-            taps_chunk<R, kChunk, 0, EXACT>(acc, xs, H + lane_base - kc - kChunk, bs + kc);
-        const int rem = kcount - kc;
-        if (rem & 16) {
-            taps_chunk<R, 16, 0, EXACT>(acc, xs, H + lane_base - kc - 16, bs + kc);
-            kc += 16;
-        }
-        if (rem & 8) {
-            taps_chunk<R, 8, 0, EXACT>(acc, xs, H + lane_base - kc - 8, bs + kc);
-            kc += 8;
-        }
-        if (rem & 4) {
-            taps_chunk<R, 4, 0, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
-            kc += 4;
-        }
-        if (rem & 2) {  // kc % 4 == 0 here; window starts 2 below a 4-aligned index
-            taps_chunk<R, 2, 2, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
-            kc += 2;
-        }
-        if (rem & 1) {
-            if (kc & 2)  // kc % 4 == 2: window starts 1 below the aligned index
-                taps_chunk<R, 1, 1, EXACT>(acc, xs, H + lane_base - kc - 2, bs + kc);
-            else         // kc % 4 == 0: window starts 3 below the aligned index
-                taps_chunk<R, 1, 3, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
-        }
+        taps_from_lds<R, EXACT>(acc, xs, H, lane_base, kcount, b + ks);
     }
-
     store_outputs<R>(acc, y + row * ldy, t0 + lane_base, N, vec_y != 0);
 }
 
-// All K taps for R consecutive outputs starting at i0, window from global
-// memory.  Chunk order is k ascending (the EXACT mode relies on it).
-template <int R, bool EXACT, bool GUARD>
-__device__ __forceinline__ void taps_all_global(float (&acc)[R], const float *__restrict__ xrow,
-                                                int64_t i0, int K, int64_t N,
-                                                const float *__restrict__ b)
-{
-    int kc = 0;
-    for (; kc + kChunk <= K; kc += kChunk)
-        taps_chunk_global<R, kChunk, 0, EXACT, GUARD>(acc, xrow, i0 - kc - kChunk, N, b + kc);
-    const int rem = K - kc;
-    if (rem & 16) {
-        taps_chunk_global<R, 16, 0, EXACT, GUARD>(acc, xrow, i0 - kc - 16, N, b + kc);
-        kc += 16;
-    }
-    if (rem & 8) {
-        taps_chunk_global<R, 8, 0, EXACT, GUARD>(acc, xrow, i0 - kc - 8, N, b + kc);
-        kc += 8;
-    }
-    if (rem & 4) {
-        taps_chunk_global<R, 4, 0, EXACT, GUARD>(acc, xrow, i0 - kc - 4, N, b + kc);
-        kc += 4;
-    }
-    if (rem & 2) {
-        taps_chunk_global<R, 2, 2, EXACT, GUARD>(acc, xrow, i0 - kc - 4, N, b + kc);
-        kc += 2;
-    }
-    if (rem & 1) {
-        if (kc & 2)
-            taps_chunk_global<R, 1, 1, EXACT, GUARD>(acc, xrow, i0 - kc - 2, N, b + kc);
-        else
-            taps_chunk_global<R, 1, 3, EXACT, GUARD>(acc, xrow, i0 - kc - 4, N, b + kc);
-    }
-}
-
-template <int R>
-__device__ __forceinline__ void store_outputs(const float (&acc)[R], float *__restrict__ yrow,
-                                              int64_t i0, int64_t N, bool vec_y)
-{
-    if (vec_y && i0 + R <= N) {
-#pragma unroll
-        for (int r = 0; r < R; r += 4) {
-            f32x4 v = {acc[r], acc[r + 1], acc[r + 2], acc[r + 3]};
-            __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(yrow + i0 + r));
-        }
-    } else {
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-            if (i0 + r < N) yrow[i0 + r] = acc[r];
-    }
-}
-
-// Variant without LDS or barriers: each lane loads its own (R+K-1)-sample
-// window from global memory; neighbouring lanes' overlapping windows are
-// served by the L1/L2 caches, so HBM still sees each sample about once.
-// Interior tiles use unguarded 16-byte loads.
-template <int R, bool EXACT>
-__global__ __launch_bounds__(kThreads) void fir_direct_global_kernel(
+// Pipelined variant for K <= kPipeMaxK: one workgroup walks `tiles_per_seg`
+// consecutive tiles of a row.  The next tile's samples are fetched into NS
+// float4 registers per lane while the current tile is computed from one of
+// two LDS buffers, so HBM reads stay in flight across the compute phase.
+// One barrier per tile: buffer (it & 1) is written at the top of iteration it
+// and was last read in iteration it-2, which every lane finished before
+// passing iteration it-1's barrier.
+template <int R, bool EXACT, int NS>
+__global__ __launch_bounds__(kThreads) void fir_direct_pipe_kernel(
     const float *__restrict__ x, int64_t ldx, const float *__restrict__ b, int K, int64_t N,
-    float *__restrict__ y, int64_t ldy, int64_t row0, int tiles_per_row, int vec_x, int vec_y)
+    float *__restrict__ y, int64_t ldy, int64_t row0, int segs_per_row, int tiles_per_seg,
+    int64_t tiles_per_row, int buf_floats, int vec_x, int vec_y)
 {
+    extern __shared__ float lds[];
     constexpr int T = kThreads * R;
-    const int64_t row = row0 + blockIdx.x / tiles_per_row;
-    const int64_t t0 = (int64_t)(blockIdx.x % tiles_per_row) * T;
+    const int H = (K + 6) & ~3;
+    const int count4 = (T + H + 4) / 4;  // <= NS * kThreads (checked by the launcher)
+    const int64_t row = row0 + blockIdx.x / segs_per_row;
+    const int64_t tfirst = (int64_t)(blockIdx.x % segs_per_row) * tiles_per_seg;
+    const int ntiles = (int)min<int64_t>((int64_t)tiles_per_seg, tiles_per_row - tfirst);
     const float *__restrict__ xrow = x + row * ldx;
-    const int64_t i0 = t0 + threadIdx.x * R;
-    float acc[R];
+    float *__restrict__ yrow = y + row * ldy;
+    const int lane_base = threadIdx.x * R;
+    const bool vx = vec_x != 0;
+
+    f32x4 pre[NS];
+    auto fetch = [&](int64_t tile) {
+        const int64_t g0 = tile * T - H;
 #pragma unroll
-    for (int r = 0; r < R; ++r) acc[r] = 0.0f;
-    const bool interior = vec_x && t0 >= (int64_t)K + 3 && t0 + T + 3 <= N;
-    if (interior)
-        taps_all_global<R, EXACT, false>(acc, xrow, i0, K, N, b);
-    else
-        taps_all_global<R, EXACT, true>(acc, xrow, i0, K, N, b);
-    store_outputs<R>(acc, y + row * ldy, i0, N, vec_y != 0);
+        for (int s = 0; s < NS; ++s) {
+            const int v = threadIdx.x + s * kThreads;
+            if (v < count4) {
+                const int64_t g = g0 + 4 * (int64_t)v;
+                if (vx && g >= 0 && g + 3 < N) {
+                    pre[s] = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g));
+                } else {
+                    pre[s].x = (g + 0 >= 0 && g + 0 < N) ? xrow[g + 0] : 0.0f;
+                    pre[s].y = (g + 1 >= 0 && g + 1 < N) ? xrow[g + 1] : 0.0f;
+                    pre[s].z = (g + 2 >= 0 && g + 2 < N) ? xrow[g + 2] : 0.0f;
+                    pre[s].w = (g + 3 >= 0 && g + 3 < N) ? xrow[g + 3] : 0.0f;
+                }
+            }
+        }
+    };
+
+    fetch(tfirst);
+    for (int it = 0; it < ntiles; ++it) {
+        float *buf = lds + (it & 1) * buf_floats;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int v = threadIdx.x + s * kThreads;
+            if (v < count4) *reinterpret_cast<f32x4 *>(buf + pad16(4 * v)) = pre[s];
+        }
+        __syncthreads();
+        if (it + 1 < ntiles) fetch(tfirst + it + 1);
+        float acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = 0.0f;
+        taps_from_lds<R, EXACT>(acc, buf, H, lane_base, K, b);
+        store_outputs<R>(acc, yrow, (tfirst + it) * T + lane_base, N, vec_y != 0);
+    }
 }
 
-template <int R, bool EXACT>
-int launch_direct_global(const float *x, int64_t ldx, const float *b, int K, int64_t B,
-                         int64_t N, float *y, int64_t ldy, hipStream_t stream)
-{
-    constexpr int T = kThreads * R;
-    const int64_t tiles = (N + T - 1) / T;
-    if (tiles > 0x7fffffff)
-        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row too long (N=%lld)", (long long)N);
-    const int vec_x = ((reinterpret_cast<uintptr_t>(x) & 15) == 0) && (ldx % 4 == 0);
-    const int vec_y = ((reinterpret_cast<uintptr_t>(y) & 15) == 0) && (ldy % 4 == 0);
-    const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / tiles);
-    for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
-        const int64_t rows = std::min(rows_per_launch, B - r0);
-        hipLaunchKernelGGL((fir_direct_global_kernel<R, EXACT>), dim3((unsigned)(rows * tiles)),
-                           dim3(kThreads), 0, stream, x, ldx, b, K, N, y, ldy, r0, (int)tiles,
-                           vec_x, vec_y);
-        SCIR_TRY(check_launch("fir_direct_global_kernel"));
-    }
-    return SCIR_HIP_OK;
+constexpr int kPipeMaxK = 1017;  // (K + 6) & ~3 <= 1020, so T + H + 4 <= T + 1024
+
+int align_flags(const float *p, int64_t ld) {
+    return ((reinterpret_cast<uintptr_t>(p) & 15) == 0) && (ld % 4 == 0);
 }
 
 template <int R, bool EXACT>
@@ -310,8 +269,7 @@ int launch_direct(const float *x, int64_t ldx, const float *b, int K, int64_t B,
     const int64_t tiles = (N + T - 1) / T;
     if (tiles > 0x7fffffff)
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row too long (N=%lld)", (long long)N);
-    const int vec_x = ((reinterpret_cast<uintptr_t>(x) & 15) == 0) && (ldx % 4 == 0);
-    const int vec_y = ((reinterpret_cast<uintptr_t>(y) & 15) == 0) && (ldy % 4 == 0);
+    const int vec_x = align_flags(x, ldx), vec_y = align_flags(y, ldy);
     // One launch covers as many rows as fit in a 2^31-1 workgroup grid.
     const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / tiles);
     for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
@@ -320,6 +278,36 @@ int launch_direct(const float *x, int64_t ldx, const float *b, int K, int64_t B,
                            dim3(kThreads), lds, stream, x, ldx, b, K, N, y, ldy, r0, (int)tiles,
                            vec_x, vec_y);
         SCIR_TRY(check_launch("fir_direct_kernel"));
+    }
+    return SCIR_HIP_OK;
+}
+
+template <int R, bool EXACT>
+int launch_direct_pipe(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
+                       float *y, int64_t ldy, int seg_tiles, hipStream_t stream)
+{
+    constexpr int T = kThreads * R;
+    constexpr int NS = (T + 1024 + kThreads * 4 - 1) / (kThreads * 4);
+    const int H = (K + 6) & ~3;
+    const int staged = T + H + 4;
+    if (K > kPipeMaxK || (staged / 4) > NS * kThreads)
+        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "pipelined direct kernel needs K <= %d",
+                    kPipeMaxK);
+    const int buf_floats = ((staged + (staged / 16) * 4 + 4) + 3) & ~3;
+    const size_t lds = 2 * (size_t)buf_floats * sizeof(float);
+    const int64_t tiles = (N + T - 1) / T;
+    const int64_t tps = std::max<int64_t>(1, std::min<int64_t>(seg_tiles, tiles));
+    const int64_t segs = (tiles + tps - 1) / tps;
+    if (segs > 0x7fffffff)
+        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row too long (N=%lld)", (long long)N);
+    const int vec_x = align_flags(x, ldx), vec_y = align_flags(y, ldy);
+    const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / segs);
+    for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
+        const int64_t rows = std::min(rows_per_launch, B - r0);
+        hipLaunchKernelGGL((fir_direct_pipe_kernel<R, EXACT, NS>), dim3((unsigned)(rows * segs)),
+                           dim3(kThreads), lds, stream, x, ldx, b, K, N, y, ldy, r0, (int)segs,
+                           (int)tps, tiles, buf_floats, vec_x, vec_y);
+        SCIR_TRY(check_launch("fir_direct_pipe_kernel"));
     }
     return SCIR_HIP_OK;
 }
@@ -333,12 +321,13 @@ int fir_direct_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B
     if (K == 0) return fill_zero_2d(y, ldy, B, N, stream);
     const TuneChoice tc = tune_lookup(K, N);
     const bool r8 = (tc.direct_r == 8) || N <= 2048;
-    if (tc.direct_loader == 1) {
+    if (tc.direct_loader == 1 && K <= kPipeMaxK) {
+        const int st = tc.direct_seg_tiles;
         if (exact)
-            return r8 ? launch_direct_global<8, true>(x, ldx, b, K, B, N, y, ldy, stream)
-                      : launch_direct_global<16, true>(x, ldx, b, K, B, N, y, ldy, stream);
-        return r8 ? launch_direct_global<8, false>(x, ldx, b, K, B, N, y, ldy, stream)
-                  : launch_direct_global<16, false>(x, ldx, b, K, B, N, y, ldy, stream);
+            return r8 ? launch_direct_pipe<8, true>(x, ldx, b, K, B, N, y, ldy, st, stream)
+                      : launch_direct_pipe<16, true>(x, ldx, b, K, B, N, y, ldy, st, stream);
+        return r8 ? launch_direct_pipe<8, false>(x, ldx, b, K, B, N, y, ldy, st, stream)
+                  : launch_direct_pipe<16, false>(x, ldx, b, K, B, N, y, ldy, st, stream);
     }
     if (exact)
         return r8 ? launch_direct<8, true>(x, ldx, b, K, B, N, y, ldy, stream)

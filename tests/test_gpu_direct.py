@@ -32,16 +32,19 @@ def host_fir(hip, x, b, precision):
     return y
 
 
-@pytest.fixture(params=[(0, 16), (0, 8), (1, 16), (1, 8)], ids=lambda p: f"loader{p[0]}-r{p[1]}")
+@pytest.fixture(params=[(0, 16, 1), (0, 8, 1), (1, 16, 8), (1, 8, 3), (1, 8, 1)],
+                ids=lambda p: f"loader{p[0]}-r{p[1]}-seg{p[2]}")
 def variant(request, hip):
-    """Each direct-kernel variant (LDS-staged / global window, 8 or 16 outputs per lane)."""
-    loader, r = request.param
+    """Each direct-kernel variant: one tile per workgroup or pipelined row
+    segments of `seg` tiles, 8 or 16 outputs per lane."""
+    loader, r, seg = request.param
     L = hip.lib()
     hip.check(L.scir_hip_tune_set(b"direct.loader", loader))
     hip.check(L.scir_hip_tune_set(b"direct.r", r))
+    hip.check(L.scir_hip_tune_set(b"direct.seg_tiles", seg))
     yield request.param
-    hip.check(L.scir_hip_tune_set(b"direct.loader", -1))
-    hip.check(L.scir_hip_tune_set(b"direct.r", -1))
+    for key in (b"direct.loader", b"direct.r", b"direct.seg_tiles"):
+        hip.check(L.scir_hip_tune_set(key, -1))
 
 
 def test_reference_kat(hip):
