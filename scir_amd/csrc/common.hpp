@@ -92,9 +92,9 @@ int fill_zero_2d(float *y, int64_t ldy, int64_t B, int64_t N, hipStream_t stream
 
 // Autotuned choices (defaults until a table is loaded or a search ran).
 struct TuneChoice {
-    int direct_r = 16;        // outputs per lane in the direct kernel
-    int direct_loader = 0;    // 0: one tile per workgroup, 1: pipelined row segments
-    int direct_seg_tiles = 8; // tiles per workgroup of the pipelined variant
+    int direct_r = 8;         // outputs per lane in the direct kernel
+    int direct_loader = 1;    // 0: one tile per workgroup, 1: pipelined row segments
+    int direct_seg_tiles = 2; // tiles per workgroup of the pipelined variant
     int mfma_tiles = 2;       // 32x32 output tiles per wave in the MFMA kernel
     int auto_mfma_min_k = 48; // AUTO uses BF16x3 MFMA from this K up
 };

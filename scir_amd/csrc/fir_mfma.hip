@@ -1,0 +1,326 @@
+// Long-tap batched FIR as a Toeplitz-block contraction on gfx950 MFMA
+// (v_mfma_f32_32x32x16_{bf16,f16}), FP32 accumulation.
+//
+// Math (lfilter order, y[n] = sum_k b[k] x[n-k]).  Cut each row's output into
+// blocks of 32 samples; block p is y[32p .. 32p+31].  With half-steps
+// m = -1 .. m_hi (m_hi = ceil((K-1)/16)):
+//   A_m[r][c] = b[16m + r - c]            (32 x 16, zero outside [0, K))
+//   B_m[c][p] = x[32p - 16m + c]          (16 x 32 block columns)
+//   y[32p + r] = sum_m (A_m B_m)[r][p]
+// Each (k, output) pair appears exactly once (m = ceil((k-r)/16), c = 16m-k+r).
+// A_m depends only on the taps: it is shared by every row and tile and is
+// prepared once per call (fir_mfma_frags_kernel) as per-lane MFMA fragments.
+// B_m is a shifted view of the same staged x: no im2col copy is made; each
+// sample is split once into LDS and read by the ~2*(K+31)/32 half-steps that
+// need it.
+//
+// Precision modes (template S = number of split terms):
+//   S = 3 (BF16x3): v = hi + mid + lo, each bf16 (hi = rn(v), mid = rn(v-hi),
+//     lo = rn(v-hi-mid); both subtractions are exact), for x and for b.  Six
+//     products are kept: hh (own accumulator), hm, mh, mm, hl, lh (a second
+//     accumulator); the dropped ml, lm, ll are below 2^-24 relative.
+//   S = 1 (FP16): x and b rounded to fp16, one product.
+//
+// Work decomposition: a workgroup (4 waves) owns a row segment of
+// 4*TW*32 blocks; each wave owns TW 32x32 output tiles (TW*1024 outputs).
+// The segment's inputs plus 16*m_hi samples of history are split into LDS
+// once.  LDS layout per split: unit u = 8 consecutive samples (16 B) at byte
+// 16*(u + u/4); a wave's B read has a 4-unit lane stride, which this padding
+// makes bank-conflict free.
+//
+// MFMA 32x32x16 operand layout (lane l = 32*g + i, g = l/32, i = l%32):
+//   A: lane holds A[i][8g + e], e = 0..7;  B: lane holds B[8g + e][i];
+//   D: lane holds D[8*(q) + 4g + j][i] in acc[4q + j], q, j = 0..3.
+#include "common.hpp"
+
+#include <algorithm>
+
+namespace scir {
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kWaves = 4;
+constexpr int kThreads = 64 * kWaves;
+constexpr int kMaxLds = 160 * 1024;
+
+inline int mfma_m_hi(int K) { return (K + 14) / 16; }  // ceil((K-1)/16) for K >= 1
+inline int mfma_steps(int K) { return mfma_m_hi(K) + 2; }
+
+__device__ __forceinline__ int unit_addr(int u) { return (u + (u >> 2)) * 16; }
+inline size_t unit_bytes(int units) { return (size_t)(units + (units >> 2) + 1) * 16; }
+
+__device__ __forceinline__ unsigned short bf16_bits(__bf16 v) { return __builtin_bit_cast(unsigned short, v); }
+__device__ __forceinline__ unsigned short f16_bits(_Float16 v) { return __builtin_bit_cast(unsigned short, v); }
+
+// Split term s (0 = hi, 1 = mid, 2 = lo) of v, as bf16 bits.
+__device__ __forceinline__ void split3(float v, unsigned short &h, unsigned short &m,
+                                       unsigned short &l)
+{
+    __bf16 bh = (__bf16)v;
+    float r1 = v - (float)bh;
+    __bf16 bm = (__bf16)r1;
+    float r2 = r1 - (float)bm;
+    __bf16 bl = (__bf16)r2;
+    h = bf16_bits(bh);
+    m = bf16_bits(bm);
+    l = bf16_bits(bl);
+}
+
+__device__ __forceinline__ unsigned int pack2(unsigned short lo, unsigned short hi)
+{
+    return (unsigned int)lo | ((unsigned int)hi << 16);
+}
+
+// Tap fragments: frags[(mi*S + s)*64 + lane] holds the 8 bf16/fp16 values of
+// A_m (m = mi - 1) for `lane`, split term s.
+template <int S>
+__global__ void fir_mfma_frags_kernel(const float *__restrict__ b, int K, int msteps,
+                                      u32x4 *__restrict__ frags)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= msteps * 64) return;
+    const int lane = idx & 63, mi = idx >> 6, m = mi - 1;
+    const int r = lane & 31, g = lane >> 5;
+    unsigned short t[S][8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int k = 16 * m + r - 8 * g - e;
+        const float v = (k >= 0 && k < K) ? b[k] : 0.0f;
+        if (S == 3) {
+            split3(v, t[0][e], t[S > 1 ? 1 : 0][e], t[S > 2 ? 2 : 0][e]);
+        } else {
+            t[0][e] = f16_bits((_Float16)v);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        u32x4 w = {pack2(t[s][0], t[s][1]), pack2(t[s][2], t[s][3]), pack2(t[s][4], t[s][5]),
+                   pack2(t[s][6], t[s][7])};
+        frags[(mi * S + s) * 64 + lane] = w;
+    }
+}
+
+template <int S>
+__device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c)
+{
+    if (S == 3)
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                       __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a),
+                                                  __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+}
+
+// Workgroup id -> logical segment id so that the workgroups one XCD receives
+// (hardware dispatches id b to XCD b % 8) cover a contiguous range of
+// segments: neighbouring segments share their input history in that XCD's L2.
+__device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb)
+{
+    const int64_t xcd = bid & 7, within = bid >> 3;
+    const int64_t q = nb >> 3, rem = nb & 7;
+    return xcd * q + (xcd < rem ? xcd : rem) + within;
+}
+
+template <int S, int TW>
+__global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
+    const float *__restrict__ x, int64_t ldx, int64_t N, float *__restrict__ y, int64_t ldy,
+    const u32x4 *__restrict__ frags, int msteps, int m_hi, int64_t row0, int segs_per_row,
+    int64_t nblocks, int split_stride, int vec_x, int vec_y)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    constexpr int SEG_BLOCKS = kWaves * TW * 32;
+    const int64_t lid = xcd_remap(blockIdx.x, nblocks);
+    const int64_t row = row0 + lid / segs_per_row;
+    const int64_t P0 = (lid % segs_per_row) * SEG_BLOCKS;  // first output block of the segment
+    const float *__restrict__ xrow = x + row * ldx;
+    const int64_t xb = 32 * P0 - 16 * (int64_t)m_hi;        // sample index of LDS unit 0
+    const int units = SEG_BLOCKS * 4 + 2 * m_hi;
+
+    // ---- stage: x[xb .. xb + 8*units) -> S split arrays in LDS ----
+    for (int u = threadIdx.x; u < units; u += kThreads) {
+        const int64_t g0 = xb + 8 * (int64_t)u;
+        float v[8];
+        if (vec_x && g0 >= 0 && g0 + 7 < N) {
+            f32x4 a = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0));
+            f32x4 c = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0 + 4));
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+            v[4] = c.x; v[5] = c.y; v[6] = c.z; v[7] = c.w;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                v[e] = (g0 + e >= 0 && g0 + e < N) ? xrow[g0 + e] : 0.0f;
+        }
+        unsigned short t[S][8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if (S == 3)
+                split3(v[e], t[0][e], t[S > 1 ? 1 : 0][e], t[S > 2 ? 2 : 0][e]);
+            else
+                t[0][e] = f16_bits((_Float16)v[e]);
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            u32x4 w = {pack2(t[s][0], t[s][1]), pack2(t[s][2], t[s][3]),
+                       pack2(t[s][4], t[s][5]), pack2(t[s][6], t[s][7])};
+            *reinterpret_cast<u32x4 *>(lds + s * split_stride + unit_addr(u)) = w;
+        }
+    }
+    __syncthreads();
+
+    // ---- contraction ----
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int p = lane & 31, g = lane >> 5;
+    f32x16 acc0[TW], acc1[TW];
+#pragma unroll
+    for (int t = 0; t < TW; ++t) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            acc0[t][i] = 0.0f;
+            acc1[t][i] = 0.0f;
+        }
+    }
+    // unit of B_m for tile t: 4*(tile block offset + p) + 2*(m_hi - m) + g
+    int ubase[TW];
+#pragma unroll
+    for (int t = 0; t < TW; ++t) ubase[t] = 4 * ((wave * TW + t) * 32 + p) + g + 2 * m_hi + 2;
+
+    const u32x4 *__restrict__ fr = frags + lane;
+    for (int mi = 0; mi < msteps; ++mi) {
+        u32x4 a[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) a[s] = fr[(mi * S + s) * 64];
+        u32x4 bv[TW][S];
+#pragma unroll
+        for (int t = 0; t < TW; ++t) {
+            const int u = ubase[t] - 2 * mi;  // m = mi - 1
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                bv[t][s] = *reinterpret_cast<const u32x4 *>(lds + s * split_stride + unit_addr(u));
+        }
+#pragma unroll
+        for (int t = 0; t < TW; ++t) acc0[t] = mfma<S>(a[0], bv[t][0], acc0[t]);
+        if (S == 3) {
+#pragma unroll
+            for (int t = 0; t < TW; ++t) {
+                acc1[t] = mfma<S>(a[0], bv[t][S > 1 ? 1 : 0], acc1[t]);   // hi(b) * mid(x)
+                acc1[t] = mfma<S>(a[S > 1 ? 1 : 0], bv[t][0], acc1[t]);   // mid(b) * hi(x)
+                acc1[t] = mfma<S>(a[S > 1 ? 1 : 0], bv[t][S > 1 ? 1 : 0], acc1[t]);  // mid*mid
+                acc1[t] = mfma<S>(a[0], bv[t][S > 2 ? 2 : 0], acc1[t]);   // hi(b) * lo(x)
+                acc1[t] = mfma<S>(a[S > 2 ? 2 : 0], bv[t][0], acc1[t]);   // lo(b) * hi(x)
+            }
+        }
+    }
+
+    // ---- epilogue: y[32*(block) + 8q + 4g + j] = acc[4q + j] ----
+    float *__restrict__ yrow = y + row * ldy;
+#pragma unroll
+    for (int t = 0; t < TW; ++t) {
+        const int64_t n0 = 32 * (P0 + (wave * TW + t) * 32 + p) + 4 * g;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            f32x4 v;
+            v.x = acc0[t][4 * q + 0] + acc1[t][4 * q + 0];
+            v.y = acc0[t][4 * q + 1] + acc1[t][4 * q + 1];
+            v.z = acc0[t][4 * q + 2] + acc1[t][4 * q + 2];
+            v.w = acc0[t][4 * q + 3] + acc1[t][4 * q + 3];
+            const int64_t n = n0 + 8 * q;
+            if (vec_y && n + 3 < N) {
+                __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(yrow + n));
+            } else {
+                if (n + 0 < N) yrow[n + 0] = v.x;
+                if (n + 1 < N) yrow[n + 1] = v.y;
+                if (n + 2 < N) yrow[n + 2] = v.z;
+                if (n + 3 < N) yrow[n + 3] = v.w;
+            }
+        }
+    }
+}
+
+int align_ok(const float *p, int64_t ld)
+{
+    return ((reinterpret_cast<uintptr_t>(p) & 15) == 0) && (ld % 4 == 0);
+}
+
+template <int S, int TW>
+int launch_mfma(const float *x, int64_t ldx, const u32x4 *frags, int K, int64_t B, int64_t N,
+                float *y, int64_t ldy, hipStream_t stream)
+{
+    constexpr int SEG_BLOCKS = kWaves * TW * 32;
+    const int m_hi = mfma_m_hi(K), msteps = mfma_steps(K);
+    const int units = SEG_BLOCKS * 4 + 2 * m_hi;
+    const int split_stride = (int)unit_bytes(units);
+    const size_t lds = (size_t)S * split_stride;
+    if (lds > (size_t)kMaxLds)
+        return fail(SCIR_HIP_ERR_UNSUPPORTED, "K=%d needs %zu bytes of LDS for the MFMA path", K,
+                    lds);
+    static bool attr_set = false;  // guarded by the device mutex held by the caller
+    if (!attr_set) {
+        SCIR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&fir_mfma_kernel<S, TW>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds));
+        attr_set = true;
+    }
+    const int64_t seg_len = 32 * (int64_t)SEG_BLOCKS;
+    const int64_t segs = (N + seg_len - 1) / seg_len;
+    if (segs > 0x7fffffff)
+        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row too long (N=%lld)", (long long)N);
+    const int vec_x = align_ok(x, ldx), vec_y = align_ok(y, ldy);
+    const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / segs);
+    for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
+        const int64_t rows = std::min(rows_per_launch, B - r0);
+        const int64_t nb = rows * segs;
+        hipLaunchKernelGGL((fir_mfma_kernel<S, TW>), dim3((unsigned)nb), dim3(kThreads), lds,
+                           stream, x, ldx, N, y, ldy, frags, msteps, m_hi, r0, (int)segs, nb,
+                           split_stride, vec_x, vec_y);
+        SCIR_TRY(check_launch("fir_mfma_kernel"));
+    }
+    return SCIR_HIP_OK;
+}
+
+template <int S>
+int launch_frags(const float *b, int K, u32x4 *frags, hipStream_t stream)
+{
+    const int msteps = mfma_steps(K);
+    const int threads = msteps * 64;
+    hipLaunchKernelGGL((fir_mfma_frags_kernel<S>), dim3((threads + 255) / 256), dim3(256), 0,
+                       stream, b, K, msteps, frags);
+    return check_launch("fir_mfma_frags_kernel");
+}
+
+template <int S>
+int run_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N, float *y,
+             int64_t ldy, hipStream_t stream, DeviceState *st)
+{
+    const int msteps = mfma_steps(K);
+    StreamScratch &sc = scratch_for(st, stream);
+    SCIR_TRY(ensure_buffer(sc.tap_frags, (size_t)msteps * S * 64 * sizeof(u32x4), stream));
+    u32x4 *frags = static_cast<u32x4 *>(sc.tap_frags.ptr);
+    SCIR_TRY(launch_frags<S>(b, K, frags, stream));
+    int tw = tune_lookup(K, N).mfma_tiles;
+    // Short rows: smaller segments keep the grid full.
+    if (N <= 4096) tw = 1;
+    switch (tw) {
+    case 1: return launch_mfma<S, 1>(x, ldx, frags, K, B, N, y, ldy, stream);
+    case 4: return launch_mfma<S, 4>(x, ldx, frags, K, B, N, y, ldy, stream);
+    default: return launch_mfma<S, 2>(x, ldx, frags, K, B, N, y, ldy, stream);
+    }
+}
+
+}  // namespace
+
+int fir_mfma_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
+                 float *y, int64_t ldy, int precision, hipStream_t stream, DeviceState *st)
+{
+    if (B == 0 || N == 0) return SCIR_HIP_OK;
+    if (K == 0) return fill_zero_2d(y, ldy, B, N, stream);
+    if (precision == SCIR_HIP_PREC_BF16X3)
+        return run_mfma<3>(x, ldx, b, K, B, N, y, ldy, stream, st);
+    if (precision == SCIR_HIP_PREC_FP16)
+        return run_mfma<1>(x, ldx, b, K, B, N, y, ldy, stream, st);
+    return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "precision %d is not an MFMA mode", precision);
+}
+
+}  // namespace scir

@@ -1,8 +1,5 @@
 // Temporary stubs.
 #include "common.hpp"
-namespace scir {
-int fir_mfma_dev(const float *, int64_t, const float *, int, int64_t, int64_t, float *, int64_t, int, hipStream_t, DeviceState *) { return fail(SCIR_HIP_ERR_UNSUPPORTED, "mfma not built yet"); }
-}
 extern "C" {
 int scir_hip_ragged_plan_create(const int64_t *, const int64_t *, int64_t, int32_t, scir_hip_ragged_plan *) { return scir::fail(SCIR_HIP_ERR_UNSUPPORTED, "ragged not built yet"); }
 int scir_hip_ragged_plan_execute(scir_hip_ragged_plan, const float *, const float *, float *, void *) { return scir::fail(SCIR_HIP_ERR_UNSUPPORTED, "ragged not built yet"); }
