@@ -47,6 +47,11 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
 constexpr int kMaxLds = 160 * 1024;
+// The main loop reads tap fragments of half-steps mi+2 and mi+3 for even mi <
+// msteps, i.e. up to index msteps+2: the fragment buffer carries this many
+// zero-filled steps past the last real one.
+constexpr int kLookahead = 4;
+constexpr int kStageBatch = 4;  // LDS units (8 samples each) loaded per lane before splitting
 
 inline int mfma_m_hi(int K) { return (K + 14) / 16; }  // ceil((K-1)/16) for K >= 1
 inline int mfma_steps(int K) { return mfma_m_hi(K) + 2; }
@@ -115,6 +120,27 @@ __device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c)
                                                   __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
 }
 
+// One half-step for TW tiles: acc0 += hi(b)*hi(x); acc1 += the five
+// correction products (BF16x3), or acc0 += b*x (FP16).
+template <int S, int TW>
+__device__ __forceinline__ void step_mfma(f32x16 (&acc0)[TW], f32x16 (&acc1)[TW],
+                                          const u32x4 (&a)[S], const u32x4 (&bv)[TW][S])
+{
+#pragma unroll
+    for (int t = 0; t < TW; ++t) acc0[t] = mfma<S>(a[0], bv[t][0], acc0[t]);
+    if (S == 3) {
+        constexpr int M = S > 1 ? 1 : 0, L = S > 2 ? 2 : 0;
+#pragma unroll
+        for (int t = 0; t < TW; ++t) {
+            acc1[t] = mfma<S>(a[0], bv[t][M], acc1[t]);  // hi(b) * mid(x)
+            acc1[t] = mfma<S>(a[M], bv[t][0], acc1[t]);  // mid(b) * hi(x)
+            acc1[t] = mfma<S>(a[M], bv[t][M], acc1[t]);  // mid(b) * mid(x)
+            acc1[t] = mfma<S>(a[0], bv[t][L], acc1[t]);  // hi(b) * lo(x)
+            acc1[t] = mfma<S>(a[L], bv[t][0], acc1[t]);  // lo(b) * hi(x)
+        }
+    }
+}
+
 // Workgroup id -> logical segment id so that the workgroups one XCD receives
 // (hardware dispatches id b to XCD b % 8) cover a contiguous range of
 // segments: neighbouring segments share their input history in that XCD's L2.
@@ -141,32 +167,45 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
     const int units = SEG_BLOCKS * 4 + 2 * m_hi;
 
     // ---- stage: x[xb .. xb + 8*units) -> S split arrays in LDS ----
-    for (int u = threadIdx.x; u < units; u += kThreads) {
-        const int64_t g0 = xb + 8 * (int64_t)u;
-        float v[8];
-        if (vec_x && g0 >= 0 && g0 + 7 < N) {
-            f32x4 a = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0));
-            f32x4 c = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0 + 4));
-            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-            v[4] = c.x; v[5] = c.y; v[6] = c.z; v[7] = c.w;
-        } else {
+    // Each lane issues the loads of kStageBatch units before splitting any of
+    // them, so their memory latencies overlap.
+    for (int u0 = threadIdx.x; u0 < units; u0 += kStageBatch * kThreads) {
+        f32x4 raw[kStageBatch][2];
 #pragma unroll
-            for (int e = 0; e < 8; ++e)
-                v[e] = (g0 + e >= 0 && g0 + e < N) ? xrow[g0 + e] : 0.0f;
+        for (int j = 0; j < kStageBatch; ++j) {
+            const int u = u0 + j * kThreads;
+            const int64_t g0 = xb + 8 * (int64_t)u;
+            if (u >= units) {
+                raw[j][0] = raw[j][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            } else if (vec_x && g0 >= 0 && g0 + 7 < N) {
+                raw[j][0] = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0));
+                raw[j][1] =
+                    __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0 + 4));
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    raw[j][e >> 2][e & 3] = (g0 + e >= 0 && g0 + e < N) ? xrow[g0 + e] : 0.0f;
+            }
         }
-        unsigned short t[S][8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            if (S == 3)
-                split3(v[e], t[0][e], t[S > 1 ? 1 : 0][e], t[S > 2 ? 2 : 0][e]);
-            else
-                t[0][e] = f16_bits((_Float16)v[e]);
-        }
+        for (int j = 0; j < kStageBatch; ++j) {
+            const int u = u0 + j * kThreads;
+            if (u >= units) break;
+            unsigned short t[S][8];
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-            u32x4 w = {pack2(t[s][0], t[s][1]), pack2(t[s][2], t[s][3]),
-                       pack2(t[s][4], t[s][5]), pack2(t[s][6], t[s][7])};
-            *reinterpret_cast<u32x4 *>(lds + s * split_stride + unit_addr(u)) = w;
+            for (int e = 0; e < 8; ++e) {
+                const float v = raw[j][e >> 2][e & 3];
+                if (S == 3)
+                    split3(v, t[0][e], t[S > 1 ? 1 : 0][e], t[S > 2 ? 2 : 0][e]);
+                else
+                    t[0][e] = f16_bits((_Float16)v);
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                u32x4 w = {pack2(t[s][0], t[s][1]), pack2(t[s][2], t[s][3]),
+                           pack2(t[s][4], t[s][5]), pack2(t[s][6], t[s][7])};
+                *reinterpret_cast<u32x4 *>(lds + s * split_stride + unit_addr(u)) = w;
+            }
         }
     }
     __syncthreads();
@@ -188,31 +227,53 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
 #pragma unroll
     for (int t = 0; t < TW; ++t) ubase[t] = 4 * ((wave * TW + t) * 32 + p) + g + 2 * m_hi + 2;
 
+    // Software pipeline: tap fragments (L2-resident global memory) are loaded
+    // two half-steps ahead, LDS operands one half-step ahead.  The fragment
+    // buffer holds kLookahead zero steps past the last one, so the look-ahead
+    // loads need no guard; LDS loads past the last step are skipped.
     const u32x4 *__restrict__ fr = frags + lane;
-    for (int mi = 0; mi < msteps; ++mi) {
-        u32x4 a[S];
+    u32x4 a_cur[2][S], a_nxt[2][S], b_cur[TW][S], b_nxt[TW][S];
 #pragma unroll
-        for (int s = 0; s < S; ++s) a[s] = fr[(mi * S + s) * 64];
-        u32x4 bv[TW][S];
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int t = 0; t < TW; ++t) {
-            const int u = ubase[t] - 2 * mi;  // m = mi - 1
+        for (int s = 0; s < S; ++s) a_cur[h][s] = fr[(h * S + s) * 64];
 #pragma unroll
-            for (int s = 0; s < S; ++s)
-                bv[t][s] = *reinterpret_cast<const u32x4 *>(lds + s * split_stride + unit_addr(u));
+    for (int t = 0; t < TW; ++t)
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            b_cur[t][s] =
+                *reinterpret_cast<const u32x4 *>(lds + s * split_stride + unit_addr(ubase[t]));
+
+    for (int mi = 0; mi < msteps; mi += 2) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int s = 0; s < S; ++s) a_nxt[h][s] = fr[((mi + 2 + h) * S + s) * 64];
+        const bool two = mi + 1 < msteps;
+        if (two) {
+#pragma unroll
+            for (int t = 0; t < TW; ++t)
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+                    b_nxt[t][s] = *reinterpret_cast<const u32x4 *>(
+                        lds + s * split_stride + unit_addr(ubase[t] - 2 * (mi + 1)));
         }
+        step_mfma<S, TW>(acc0, acc1, a_cur[0], b_cur);
+        if (two) {
+            if (mi + 2 < msteps) {
 #pragma unroll
-        for (int t = 0; t < TW; ++t) acc0[t] = mfma<S>(a[0], bv[t][0], acc0[t]);
-        if (S == 3) {
+                for (int t = 0; t < TW; ++t)
 #pragma unroll
-            for (int t = 0; t < TW; ++t) {
-                acc1[t] = mfma<S>(a[0], bv[t][S > 1 ? 1 : 0], acc1[t]);   // hi(b) * mid(x)
-                acc1[t] = mfma<S>(a[S > 1 ? 1 : 0], bv[t][0], acc1[t]);   // mid(b) * hi(x)
-                acc1[t] = mfma<S>(a[S > 1 ? 1 : 0], bv[t][S > 1 ? 1 : 0], acc1[t]);  // mid*mid
-                acc1[t] = mfma<S>(a[0], bv[t][S > 2 ? 2 : 0], acc1[t]);   // hi(b) * lo(x)
-                acc1[t] = mfma<S>(a[S > 2 ? 2 : 0], bv[t][0], acc1[t]);   // lo(b) * hi(x)
+                    for (int s = 0; s < S; ++s)
+                        b_cur[t][s] = *reinterpret_cast<const u32x4 *>(
+                            lds + s * split_stride + unit_addr(ubase[t] - 2 * (mi + 2)));
             }
+            step_mfma<S, TW>(acc0, acc1, a_cur[1], b_nxt);
         }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int s = 0; s < S; ++s) a_cur[h][s] = a_nxt[h][s];
     }
 
     // ---- epilogue: y[32*(block) + 8q + 4g + j] = acc[4q + j] ----
@@ -283,7 +344,7 @@ int launch_mfma(const float *x, int64_t ldx, const u32x4 *frags, int K, int64_t 
 template <int S>
 int launch_frags(const float *b, int K, u32x4 *frags, hipStream_t stream)
 {
-    const int msteps = mfma_steps(K);
+    const int msteps = mfma_steps(K) + kLookahead;  // zero steps for the look-ahead loads
     const int threads = msteps * 64;
     hipLaunchKernelGGL((fir_mfma_frags_kernel<S>), dim3((threads + 255) / 256), dim3(256), 0,
                        stream, b, K, msteps, frags);
@@ -296,7 +357,8 @@ int run_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int6
 {
     const int msteps = mfma_steps(K);
     StreamScratch &sc = scratch_for(st, stream);
-    SCIR_TRY(ensure_buffer(sc.tap_frags, (size_t)msteps * S * 64 * sizeof(u32x4), stream));
+    SCIR_TRY(ensure_buffer(sc.tap_frags, (size_t)(msteps + kLookahead) * S * 64 * sizeof(u32x4),
+                           stream));
     u32x4 *frags = static_cast<u32x4 *>(sc.tap_frags.ptr);
     SCIR_TRY(launch_frags<S>(b, K, frags, stream));
     int tw = tune_lookup(K, N).mfma_tiles;
