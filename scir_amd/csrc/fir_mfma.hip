@@ -141,6 +141,31 @@ __device__ __forceinline__ void step_mfma(f32x16 (&acc0)[TW], f32x16 (&acc1)[TW]
     }
 }
 
+// Tap fragments of half-steps mi and mi+1.
+template <int S>
+__device__ __forceinline__ void load_a(u32x4 (&a)[2][S], const u32x4 *__restrict__ fr, int mi)
+{
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int s = 0; s < S; ++s) a[h][s] = fr[((mi + h) * S + s) * 64];
+}
+
+// B operands of half-step mi for TW tiles (unit clamped to 0 for look-ahead
+// steps past the last one, whose values are never used).
+template <int S, int TW>
+__device__ __forceinline__ void load_b(u32x4 (&bv)[TW][S], const unsigned char *lds,
+                                       int split_stride, const int (&ubase)[TW], int mi)
+{
+#pragma unroll
+    for (int t = 0; t < TW; ++t) {
+        const int u = max(ubase[t] - 2 * mi, 0);
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            bv[t][s] = *reinterpret_cast<const u32x4 *>(lds + s * split_stride + unit_addr(u));
+    }
+}
+
 // Workgroup id -> logical segment id so that the workgroups one XCD receives
 // (hardware dispatches id b to XCD b % 8) cover a contiguous range of
 // segments: neighbouring segments share their input history in that XCD's L2.
@@ -227,53 +252,51 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
 #pragma unroll
     for (int t = 0; t < TW; ++t) ubase[t] = 4 * ((wave * TW + t) * 32 + p) + g + 2 * m_hi + 2;
 
-    // Software pipeline: tap fragments (L2-resident global memory) are loaded
-    // two half-steps ahead, LDS operands one half-step ahead.  The fragment
-    // buffer holds kLookahead zero steps past the last one, so the look-ahead
-    // loads need no guard; LDS loads past the last step are skipped.
+    // Software pipeline over half-steps, 4 per trip with no branches: tap
+    // fragments (L2-resident global memory) are fetched two half-steps ahead
+    // and LDS operands one half-step ahead, alternating between two register
+    // sets so no copies are needed.  The fragment buffer carries kLookahead
+    // zero steps past the last one and look-ahead LDS units are clamped to 0,
+    // so every load stays in bounds; the few remaining half-steps run after
+    // the loop.
     const u32x4 *__restrict__ fr = frags + lane;
-    u32x4 a_cur[2][S], a_nxt[2][S], b_cur[TW][S], b_nxt[TW][S];
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int s = 0; s < S; ++s) a_cur[h][s] = fr[(h * S + s) * 64];
-#pragma unroll
-    for (int t = 0; t < TW; ++t)
-#pragma unroll
-        for (int s = 0; s < S; ++s)
-            b_cur[t][s] =
-                *reinterpret_cast<const u32x4 *>(lds + s * split_stride + unit_addr(ubase[t]));
-
-    for (int mi = 0; mi < msteps; mi += 2) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int s = 0; s < S; ++s) a_nxt[h][s] = fr[((mi + 2 + h) * S + s) * 64];
-        const bool two = mi + 1 < msteps;
-        if (two) {
-#pragma unroll
-            for (int t = 0; t < TW; ++t)
-#pragma unroll
-                for (int s = 0; s < S; ++s)
-                    b_nxt[t][s] = *reinterpret_cast<const u32x4 *>(
-                        lds + s * split_stride + unit_addr(ubase[t] - 2 * (mi + 1)));
+    u32x4 aA[2][S], aB[2][S], bA[TW][S], bB[TW][S];
+    load_a<S>(aA, fr, 0);
+    load_b<S, TW>(bA, lds, split_stride, ubase, 0);
+    int mi = 0;
+    // sched_barrier(0) fences keep each look-ahead load group ahead of the
+    // MFMAs it must overlap (the scheduler would otherwise sink the loads next
+    // to their first use and wait on them).
+    for (; mi + 4 <= msteps; mi += 4) {
+        load_a<S>(aB, fr, mi + 2);
+        load_b<S, TW>(bB, lds, split_stride, ubase, mi + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        step_mfma<S, TW>(acc0, acc1, aA[0], bA);
+        __builtin_amdgcn_sched_barrier(0);
+        load_b<S, TW>(bA, lds, split_stride, ubase, mi + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        step_mfma<S, TW>(acc0, acc1, aA[1], bB);
+        __builtin_amdgcn_sched_barrier(0);
+        load_a<S>(aA, fr, mi + 4);
+        load_b<S, TW>(bB, lds, split_stride, ubase, mi + 3);
+        __builtin_amdgcn_sched_barrier(0);
+        step_mfma<S, TW>(acc0, acc1, aB[0], bA);
+        __builtin_amdgcn_sched_barrier(0);
+        load_b<S, TW>(bA, lds, split_stride, ubase, mi + 4);
+        __builtin_amdgcn_sched_barrier(0);
+        step_mfma<S, TW>(acc0, acc1, aB[1], bB);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // aA holds half-steps mi, mi+1 and bA half-step mi.
+    if (mi < msteps) {
+        load_b<S, TW>(bB, lds, split_stride, ubase, mi + 1);
+        step_mfma<S, TW>(acc0, acc1, aA[0], bA);
+        if (mi + 1 < msteps) {
+            load_a<S>(aB, fr, mi + 2);
+            load_b<S, TW>(bA, lds, split_stride, ubase, mi + 2);
+            step_mfma<S, TW>(acc0, acc1, aA[1], bB);
+            if (mi + 2 < msteps) step_mfma<S, TW>(acc0, acc1, aB[0], bA);
         }
-        step_mfma<S, TW>(acc0, acc1, a_cur[0], b_cur);
-        if (two) {
-            if (mi + 2 < msteps) {
-#pragma unroll
-                for (int t = 0; t < TW; ++t)
-#pragma unroll
-                    for (int s = 0; s < S; ++s)
-                        b_cur[t][s] = *reinterpret_cast<const u32x4 *>(
-                            lds + s * split_stride + unit_addr(ubase[t] - 2 * (mi + 2)));
-            }
-            step_mfma<S, TW>(acc0, acc1, a_cur[1], b_nxt);
-        }
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int s = 0; s < S; ++s) a_cur[h][s] = a_nxt[h][s];
     }
 
     // ---- epilogue: y[32*(block) + 8q + 4g + j] = acc[4q + j] ----
