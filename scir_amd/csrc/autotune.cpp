@@ -18,7 +18,7 @@ TuneChoice tune_lookup(int, int64_t)
     if (g_over_direct_r >= 0) c.direct_r = g_over_direct_r;
     if (g_over_direct_loader >= 0) c.direct_loader = g_over_direct_loader;
     if (g_over_direct_seg >= 0) c.direct_seg_tiles = g_over_direct_seg;
-    if (g_over_mfma_tiles >= 0) c.mfma_tiles = g_over_mfma_tiles;
+    if (g_over_mfma_tiles >= 0) c.mfma_tiles = c.mfma_tiles_fp16 = g_over_mfma_tiles;
     if (g_over_auto_min_k >= 0) c.auto_mfma_min_k = g_over_auto_min_k;
     return c;
 }

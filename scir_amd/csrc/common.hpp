@@ -84,10 +84,17 @@ int check_dense_args(const void *x, int64_t ldx, const void *b, int32_t K, int64
 int check_launch(const char *what);
 
 // ---- dense entry points shared across translation units -----------------
+// `rev` runs the filter back to front: y[N-1-i] = sum_k b[k] x[N-1-i+k], the
+// anti-causal correlation that gives dL/dx.
 int fir_direct_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
-                   float *y, int64_t ldy, bool exact, hipStream_t stream, DeviceState *st);
+                   float *y, int64_t ldy, bool exact, bool rev, hipStream_t stream,
+                   DeviceState *st);
 int fir_mfma_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
-                 float *y, int64_t ldy, int precision, hipStream_t stream, DeviceState *st);
+                 float *y, int64_t ldy, int precision, bool rev, hipStream_t stream,
+                 DeviceState *st);
+int fir_dense_dispatch(const float *x, int64_t ldx, const float *b, int32_t K, int64_t B,
+                       int64_t N, float *y, int64_t ldy, int32_t precision, bool rev,
+                       hipStream_t stream, DeviceState *st);
 int fill_zero_2d(float *y, int64_t ldy, int64_t B, int64_t N, hipStream_t stream);
 
 // Autotuned choices (defaults until a table is loaded or a search ran).
@@ -95,8 +102,9 @@ struct TuneChoice {
     int direct_r = 8;         // outputs per lane in the direct kernel
     int direct_loader = 1;    // 0: one tile per workgroup, 1: pipelined row segments
     int direct_seg_tiles = 2; // tiles per workgroup of the pipelined variant
-    int mfma_tiles = 2;       // 32x32 output tiles per wave in the MFMA kernel
-    int auto_mfma_min_k = 48; // AUTO uses BF16x3 MFMA from this K up
+    int mfma_tiles = 2;       // 32x32 output tiles per wave, BF16x3 MFMA kernel
+    int mfma_tiles_fp16 = 4;  // the same for the FP16 MFMA kernel
+    int auto_mfma_min_k = 160; // AUTO uses BF16x3 MFMA from this K up
 };
 TuneChoice tune_lookup(int K, int64_t N);
 

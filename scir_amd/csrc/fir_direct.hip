@@ -88,24 +88,39 @@ __device__ __forceinline__ void taps_chunk(float (&acc)[R], const float *xs, int
     }
 }
 
-// Stages x[row, g0 .. g0 + count) (count a multiple of 4, g0 a multiple of 4)
-// into xs (padded layout); samples outside [0, N) read as zero.
-__device__ __forceinline__ void stage_row(float *xs, const float *__restrict__ xrow, int64_t g0,
-                                          int count, int64_t N, bool vec_ok)
+// Logical samples g..g+3 of a row as a float4, zero outside [0, N).  With
+// `rev` the row is read back to front (logical i == physical N-1-i), which
+// turns the causal filter into the anti-causal correlation of the adjoint.
+// `vec` promises 16-byte aligned rows (and N % 4 == 0 when rev is set).
+__device__ __forceinline__ f32x4 load4(const float *__restrict__ xrow, int64_t g, int64_t N,
+                                       bool vec, bool rev)
 {
-    for (int v = threadIdx.x; v < count / 4; v += kThreads) {
-        int64_t g = g0 + 4 * (int64_t)v;
-        f32x4 val;
-        if (vec_ok && g >= 0 && g + 3 < N) {
-            val = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g));
+    f32x4 v;
+    if (vec && g >= 0 && g + 3 < N) {
+        if (!rev) {
+            v = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g));
         } else {
-            val.x = (g + 0 >= 0 && g + 0 < N) ? xrow[g + 0] : 0.0f;
-            val.y = (g + 1 >= 0 && g + 1 < N) ? xrow[g + 1] : 0.0f;
-            val.z = (g + 2 >= 0 && g + 2 < N) ? xrow[g + 2] : 0.0f;
-            val.w = (g + 3 >= 0 && g + 3 < N) ? xrow[g + 3] : 0.0f;
+            f32x4 t = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + (N - 4 - g)));
+            v = f32x4{t.w, t.z, t.y, t.x};
         }
-        *reinterpret_cast<f32x4 *>(xs + pad16(4 * v)) = val;
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int64_t i = g + c;
+            v[c] = (i >= 0 && i < N) ? xrow[rev ? N - 1 - i : i] : 0.0f;
+        }
     }
+    return v;
+}
+
+// Stages logical samples g0 .. g0 + count (count and g0 multiples of 4) into
+// xs (padded layout).
+__device__ __forceinline__ void stage_row(float *xs, const float *__restrict__ xrow, int64_t g0,
+                                          int count, int64_t N, bool vec_ok, bool rev)
+{
+    for (int v = threadIdx.x; v < count / 4; v += kThreads)
+        *reinterpret_cast<f32x4 *>(xs + pad16(4 * v)) =
+            load4(xrow, g0 + 4 * (int64_t)v, N, vec_ok, rev);
 }
 
 // acc[r] += sum_{k' < kcount} bs[k'] * x[i_r - k'] with the window in LDS:
@@ -145,18 +160,25 @@ __device__ __forceinline__ void taps_from_lds(float (&acc)[R], const float *xs, 
 
 template <int R>
 __device__ __forceinline__ void store_outputs(const float (&acc)[R], float *__restrict__ yrow,
-                                              int64_t i0, int64_t N, bool vec_y)
+                                              int64_t i0, int64_t N, bool vec_y, bool rev)
 {
     if (vec_y && i0 + R <= N) {
 #pragma unroll
         for (int r = 0; r < R; r += 4) {
-            f32x4 v = {acc[r], acc[r + 1], acc[r + 2], acc[r + 3]};
-            __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(yrow + i0 + r));
+            if (!rev) {
+                f32x4 v = {acc[r], acc[r + 1], acc[r + 2], acc[r + 3]};
+                __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(yrow + i0 + r));
+            } else {
+                f32x4 v = {acc[r + 3], acc[r + 2], acc[r + 1], acc[r]};
+                __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(yrow + (N - 4 - i0 - r)));
+            }
         }
     } else {
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-            if (i0 + r < N) yrow[i0 + r] = acc[r];
+        for (int r = 0; r < R; ++r) {
+            const int64_t i = i0 + r;
+            if (i < N) yrow[rev ? N - 1 - i : i] = acc[r];
+        }
     }
 }
 
@@ -164,7 +186,8 @@ __device__ __forceinline__ void store_outputs(const float (&acc)[R], float *__re
 template <int R, bool EXACT>
 __global__ __launch_bounds__(kThreads) void fir_direct_kernel(
     const float *__restrict__ x, int64_t ldx, const float *__restrict__ b, int K, int64_t N,
-    float *__restrict__ y, int64_t ldy, int64_t row0, int tiles_per_row, int vec_x, int vec_y)
+    float *__restrict__ y, int64_t ldy, int64_t row0, int tiles_per_row, int vec_x, int vec_y,
+    int rev)
 {
     extern __shared__ float xs[];
     constexpr int T = kThreads * R;
@@ -181,11 +204,11 @@ __global__ __launch_bounds__(kThreads) void fir_direct_kernel(
         const int kcount = min(kSuper, K - ks);
         const int H = (kcount + 6) & ~3;  // >= kcount + 3, multiple of 4
         if (ks > 0) __syncthreads();      // previous super-chunk done with xs
-        stage_row(xs, xrow, t0 - ks - H, T + H + 4, N, vec_x != 0);
+        stage_row(xs, xrow, t0 - ks - H, T + H + 4, N, vec_x != 0, rev != 0);
         __syncthreads();
         taps_from_lds<R, EXACT>(acc, xs, H, lane_base, kcount, b + ks);
     }
-    store_outputs<R>(acc, y + row * ldy, t0 + lane_base, N, vec_y != 0);
+    store_outputs<R>(acc, y + row * ldy, t0 + lane_base, N, vec_y != 0, rev != 0);
 }
 
 // Pipelined variant for K <= kPipeMaxK: one workgroup walks `tiles_per_seg`
@@ -199,7 +222,7 @@ template <int R, bool EXACT, int NS>
 __global__ __launch_bounds__(kThreads) void fir_direct_pipe_kernel(
     const float *__restrict__ x, int64_t ldx, const float *__restrict__ b, int K, int64_t N,
     float *__restrict__ y, int64_t ldy, int64_t row0, int segs_per_row, int tiles_per_seg,
-    int64_t tiles_per_row, int buf_floats, int vec_x, int vec_y)
+    int64_t tiles_per_row, int buf_floats, int vec_x, int vec_y, int rev)
 {
     extern __shared__ float lds[];
     constexpr int T = kThreads * R;
@@ -219,17 +242,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_pipe_kernel(
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             const int v = threadIdx.x + s * kThreads;
-            if (v < count4) {
-                const int64_t g = g0 + 4 * (int64_t)v;
-                if (vx && g >= 0 && g + 3 < N) {
-                    pre[s] = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g));
-                } else {
-                    pre[s].x = (g + 0 >= 0 && g + 0 < N) ? xrow[g + 0] : 0.0f;
-                    pre[s].y = (g + 1 >= 0 && g + 1 < N) ? xrow[g + 1] : 0.0f;
-                    pre[s].z = (g + 2 >= 0 && g + 2 < N) ? xrow[g + 2] : 0.0f;
-                    pre[s].w = (g + 3 >= 0 && g + 3 < N) ? xrow[g + 3] : 0.0f;
-                }
-            }
+            if (v < count4) pre[s] = load4(xrow, g0 + 4 * (int64_t)v, N, vx, rev != 0);
         }
     };
 
@@ -247,7 +260,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_pipe_kernel(
 #pragma unroll
         for (int r = 0; r < R; ++r) acc[r] = 0.0f;
         taps_from_lds<R, EXACT>(acc, buf, H, lane_base, K, b);
-        store_outputs<R>(acc, yrow, (tfirst + it) * T + lane_base, N, vec_y != 0);
+        store_outputs<R>(acc, yrow, (tfirst + it) * T + lane_base, N, vec_y != 0, rev != 0);
     }
 }
 
@@ -259,7 +272,7 @@ int align_flags(const float *p, int64_t ld) {
 
 template <int R, bool EXACT>
 int launch_direct(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
-                  float *y, int64_t ldy, hipStream_t stream)
+                  float *y, int64_t ldy, bool rev, hipStream_t stream)
 {
     constexpr int T = kThreads * R;
     const int kmax = std::min(K, kSuper);
@@ -269,14 +282,15 @@ int launch_direct(const float *x, int64_t ldx, const float *b, int K, int64_t B,
     const int64_t tiles = (N + T - 1) / T;
     if (tiles > 0x7fffffff)
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row too long (N=%lld)", (long long)N);
-    const int vec_x = align_flags(x, ldx), vec_y = align_flags(y, ldy);
+    const int nvec = !rev || (N % 4 == 0);
+    const int vec_x = align_flags(x, ldx) && nvec, vec_y = align_flags(y, ldy) && nvec;
     // One launch covers as many rows as fit in a 2^31-1 workgroup grid.
     const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / tiles);
     for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
         const int64_t rows = std::min(rows_per_launch, B - r0);
         hipLaunchKernelGGL((fir_direct_kernel<R, EXACT>), dim3((unsigned)(rows * tiles)),
                            dim3(kThreads), lds, stream, x, ldx, b, K, N, y, ldy, r0, (int)tiles,
-                           vec_x, vec_y);
+                           vec_x, vec_y, (int)rev);
         SCIR_TRY(check_launch("fir_direct_kernel"));
     }
     return SCIR_HIP_OK;
@@ -284,7 +298,7 @@ int launch_direct(const float *x, int64_t ldx, const float *b, int K, int64_t B,
 
 template <int R, bool EXACT>
 int launch_direct_pipe(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
-                       float *y, int64_t ldy, int seg_tiles, hipStream_t stream)
+                       float *y, int64_t ldy, int seg_tiles, bool rev, hipStream_t stream)
 {
     constexpr int T = kThreads * R;
     constexpr int NS = (T + 1024 + kThreads * 4 - 1) / (kThreads * 4);
@@ -300,13 +314,14 @@ int launch_direct_pipe(const float *x, int64_t ldx, const float *b, int K, int64
     const int64_t segs = (tiles + tps - 1) / tps;
     if (segs > 0x7fffffff)
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row too long (N=%lld)", (long long)N);
-    const int vec_x = align_flags(x, ldx), vec_y = align_flags(y, ldy);
+    const int nvec = !rev || (N % 4 == 0);
+    const int vec_x = align_flags(x, ldx) && nvec, vec_y = align_flags(y, ldy) && nvec;
     const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / segs);
     for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
         const int64_t rows = std::min(rows_per_launch, B - r0);
         hipLaunchKernelGGL((fir_direct_pipe_kernel<R, EXACT, NS>), dim3((unsigned)(rows * segs)),
                            dim3(kThreads), lds, stream, x, ldx, b, K, N, y, ldy, r0, (int)segs,
-                           (int)tps, tiles, buf_floats, vec_x, vec_y);
+                           (int)tps, tiles, buf_floats, vec_x, vec_y, (int)rev);
         SCIR_TRY(check_launch("fir_direct_pipe_kernel"));
     }
     return SCIR_HIP_OK;
@@ -315,7 +330,7 @@ int launch_direct_pipe(const float *x, int64_t ldx, const float *b, int K, int64
 }  // namespace
 
 int fir_direct_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
-                   float *y, int64_t ldy, bool exact, hipStream_t stream, DeviceState *)
+                   float *y, int64_t ldy, bool exact, bool rev, hipStream_t stream, DeviceState *)
 {
     if (B == 0 || N == 0) return SCIR_HIP_OK;
     if (K == 0) return fill_zero_2d(y, ldy, B, N, stream);
@@ -324,36 +339,37 @@ int fir_direct_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B
     if (tc.direct_loader == 1 && K <= kPipeMaxK) {
         const int st = tc.direct_seg_tiles;
         if (exact)
-            return r8 ? launch_direct_pipe<8, true>(x, ldx, b, K, B, N, y, ldy, st, stream)
-                      : launch_direct_pipe<16, true>(x, ldx, b, K, B, N, y, ldy, st, stream);
-        return r8 ? launch_direct_pipe<8, false>(x, ldx, b, K, B, N, y, ldy, st, stream)
-                  : launch_direct_pipe<16, false>(x, ldx, b, K, B, N, y, ldy, st, stream);
+            return r8 ? launch_direct_pipe<8, true>(x, ldx, b, K, B, N, y, ldy, st, rev, stream)
+                      : launch_direct_pipe<16, true>(x, ldx, b, K, B, N, y, ldy, st, rev, stream);
+        return r8 ? launch_direct_pipe<8, false>(x, ldx, b, K, B, N, y, ldy, st, rev, stream)
+                  : launch_direct_pipe<16, false>(x, ldx, b, K, B, N, y, ldy, st, rev, stream);
     }
     if (exact)
-        return r8 ? launch_direct<8, true>(x, ldx, b, K, B, N, y, ldy, stream)
-                  : launch_direct<16, true>(x, ldx, b, K, B, N, y, ldy, stream);
-    return r8 ? launch_direct<8, false>(x, ldx, b, K, B, N, y, ldy, stream)
-              : launch_direct<16, false>(x, ldx, b, K, B, N, y, ldy, stream);
+        return r8 ? launch_direct<8, true>(x, ldx, b, K, B, N, y, ldy, rev, stream)
+                  : launch_direct<16, true>(x, ldx, b, K, B, N, y, ldy, rev, stream);
+    return r8 ? launch_direct<8, false>(x, ldx, b, K, B, N, y, ldy, rev, stream)
+              : launch_direct<16, false>(x, ldx, b, K, B, N, y, ldy, rev, stream);
 }
 
-// Dense dispatch shared by the host and device tiers.
-static int fir_dense_dispatch(const float *x, int64_t ldx, const float *b, int32_t K, int64_t B,
-                              int64_t N, float *y, int64_t ldy, int32_t precision,
-                              hipStream_t stream, DeviceState *st)
+// Dense dispatch shared by the host tier, the device tier and the dL/dx
+// adjoint (rev = true: the same filter run back to front).
+int fir_dense_dispatch(const float *x, int64_t ldx, const float *b, int32_t K, int64_t B,
+                       int64_t N, float *y, int64_t ldy, int32_t precision, bool rev,
+                       hipStream_t stream, DeviceState *st)
 {
     switch (precision) {
     case SCIR_HIP_PREC_F32:
-        return fir_direct_dev(x, ldx, b, K, B, N, y, ldy, false, stream, st);
+        return fir_direct_dev(x, ldx, b, K, B, N, y, ldy, false, rev, stream, st);
     case SCIR_HIP_PREC_F32_EXACT:
-        return fir_direct_dev(x, ldx, b, K, B, N, y, ldy, true, stream, st);
+        return fir_direct_dev(x, ldx, b, K, B, N, y, ldy, true, rev, stream, st);
     case SCIR_HIP_PREC_BF16X3:
     case SCIR_HIP_PREC_FP16:
-        return fir_mfma_dev(x, ldx, b, K, B, N, y, ldy, precision, stream, st);
+        return fir_mfma_dev(x, ldx, b, K, B, N, y, ldy, precision, rev, stream, st);
     case SCIR_HIP_PREC_AUTO: {
         const TuneChoice tc = tune_lookup(K, N);
         if (K >= tc.auto_mfma_min_k)
-            return fir_mfma_dev(x, ldx, b, K, B, N, y, ldy, SCIR_HIP_PREC_BF16X3, stream, st);
-        return fir_direct_dev(x, ldx, b, K, B, N, y, ldy, false, stream, st);
+            return fir_mfma_dev(x, ldx, b, K, B, N, y, ldy, SCIR_HIP_PREC_BF16X3, rev, stream, st);
+        return fir_direct_dev(x, ldx, b, K, B, N, y, ldy, false, rev, stream, st);
     }
     default:
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "unknown precision %d", precision);
@@ -375,7 +391,8 @@ int scir_hip_fir_f32_dev(const float *x, int64_t ldx, const float *b, int32_t K,
     DeviceState *st = nullptr;
     SCIR_TRY(current_device_state(&st));
     std::lock_guard<std::mutex> g(st->mu);
-    return fir_dense_dispatch(x, ldx, b, K, B, N, y, ldy, precision, as_stream(stream), st);
+    return fir_dense_dispatch(x, ldx, b, K, B, N, y, ldy, precision, false, as_stream(stream),
+                              st);
 }
 
 int scir_hip_fir_f32_host(const float *x, int64_t ldx, const float *b, int32_t K, int64_t B,
@@ -406,7 +423,7 @@ int scir_hip_fir_f32_host(const float *x, int64_t ldx, const float *b, int32_t K
                                   hipMemcpyHostToDevice, s));
     if (K > 0)
         SCIR_HIP_TRY(hipMemcpyAsync(db, b, (size_t)K * sizeof(float), hipMemcpyHostToDevice, s));
-    SCIR_TRY(fir_dense_dispatch(dx, ldd, db, K, B, N, dy, ldd, precision, s, st));
+    SCIR_TRY(fir_dense_dispatch(dx, ldd, db, K, B, N, dy, ldd, precision, false, s, st));
     SCIR_HIP_TRY(hipMemcpy2DAsync(y, ldy * sizeof(float), dy, ldd * sizeof(float), row_bytes, B,
                                   hipMemcpyDeviceToHost, s));
     SCIR_HIP_TRY(hipStreamSynchronize(s));

@@ -180,7 +180,7 @@ template <int S, int TW>
 __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
     const float *__restrict__ x, int64_t ldx, int64_t N, float *__restrict__ y, int64_t ldy,
     const u32x4 *__restrict__ frags, int msteps, int m_hi, int64_t row0, int segs_per_row,
-    int64_t nblocks, int split_stride, int vec_x, int vec_y)
+    int64_t nblocks, int split_stride, int vec_x, int vec_y, int rev)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     constexpr int SEG_BLOCKS = kWaves * TW * 32;
@@ -203,13 +203,25 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
             if (u >= units) {
                 raw[j][0] = raw[j][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
             } else if (vec_x && g0 >= 0 && g0 + 7 < N) {
-                raw[j][0] = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0));
-                raw[j][1] =
-                    __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0 + 4));
+                if (!rev) {
+                    raw[j][0] =
+                        __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0));
+                    raw[j][1] =
+                        __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0 + 4));
+                } else {  // logical g0..g0+7 == physical N-1-g0 .. N-8-g0
+                    const f32x4 a = __builtin_nontemporal_load(
+                        reinterpret_cast<const f32x4 *>(xrow + (N - 4 - g0)));
+                    const f32x4 c = __builtin_nontemporal_load(
+                        reinterpret_cast<const f32x4 *>(xrow + (N - 8 - g0)));
+                    raw[j][0] = f32x4{a.w, a.z, a.y, a.x};
+                    raw[j][1] = f32x4{c.w, c.z, c.y, c.x};
+                }
             } else {
 #pragma unroll
-                for (int e = 0; e < 8; ++e)
-                    raw[j][e >> 2][e & 3] = (g0 + e >= 0 && g0 + e < N) ? xrow[g0 + e] : 0.0f;
+                for (int e = 0; e < 8; ++e) {
+                    const int64_t i = g0 + e;
+                    raw[j][e >> 2][e & 3] = (i >= 0 && i < N) ? xrow[rev ? N - 1 - i : i] : 0.0f;
+                }
             }
         }
 #pragma unroll
@@ -313,12 +325,15 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
             v.w = acc0[t][4 * q + 3] + acc1[t][4 * q + 3];
             const int64_t n = n0 + 8 * q;
             if (vec_y && n + 3 < N) {
-                __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(yrow + n));
+                if (!rev)
+                    __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(yrow + n));
+                else
+                    __builtin_nontemporal_store(f32x4{v.w, v.z, v.y, v.x},
+                                                reinterpret_cast<f32x4 *>(yrow + (N - 4 - n)));
             } else {
-                if (n + 0 < N) yrow[n + 0] = v.x;
-                if (n + 1 < N) yrow[n + 1] = v.y;
-                if (n + 2 < N) yrow[n + 2] = v.z;
-                if (n + 3 < N) yrow[n + 3] = v.w;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (n + j < N) yrow[rev ? N - 1 - (n + j) : n + j] = v[j];
             }
         }
     }
@@ -331,7 +346,7 @@ int align_ok(const float *p, int64_t ld)
 
 template <int S, int TW>
 int launch_mfma(const float *x, int64_t ldx, const u32x4 *frags, int K, int64_t B, int64_t N,
-                float *y, int64_t ldy, hipStream_t stream)
+                float *y, int64_t ldy, bool rev, hipStream_t stream)
 {
     constexpr int SEG_BLOCKS = kWaves * TW * 32;
     const int m_hi = mfma_m_hi(K), msteps = mfma_steps(K);
@@ -351,14 +366,15 @@ int launch_mfma(const float *x, int64_t ldx, const u32x4 *frags, int K, int64_t 
     const int64_t segs = (N + seg_len - 1) / seg_len;
     if (segs > 0x7fffffff)
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row too long (N=%lld)", (long long)N);
-    const int vec_x = align_ok(x, ldx), vec_y = align_ok(y, ldy);
+    const int nvec = !rev || (N % 4 == 0);
+    const int vec_x = align_ok(x, ldx) && nvec, vec_y = align_ok(y, ldy) && nvec;
     const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / segs);
     for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
         const int64_t rows = std::min(rows_per_launch, B - r0);
         const int64_t nb = rows * segs;
         hipLaunchKernelGGL((fir_mfma_kernel<S, TW>), dim3((unsigned)nb), dim3(kThreads), lds,
                            stream, x, ldx, N, y, ldy, frags, msteps, m_hi, r0, (int)segs, nb,
-                           split_stride, vec_x, vec_y);
+                           split_stride, vec_x, vec_y, (int)rev);
         SCIR_TRY(check_launch("fir_mfma_kernel"));
     }
     return SCIR_HIP_OK;
@@ -376,7 +392,7 @@ int launch_frags(const float *b, int K, u32x4 *frags, hipStream_t stream)
 
 template <int S>
 int run_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N, float *y,
-             int64_t ldy, hipStream_t stream, DeviceState *st)
+             int64_t ldy, bool rev, hipStream_t stream, DeviceState *st)
 {
     const int msteps = mfma_steps(K);
     StreamScratch &sc = scratch_for(st, stream);
@@ -384,27 +400,29 @@ int run_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int6
                            stream));
     u32x4 *frags = static_cast<u32x4 *>(sc.tap_frags.ptr);
     SCIR_TRY(launch_frags<S>(b, K, frags, stream));
-    int tw = tune_lookup(K, N).mfma_tiles;
+    const TuneChoice tc = tune_lookup(K, N);
+    int tw = S == 3 ? tc.mfma_tiles : tc.mfma_tiles_fp16;
     // Short rows: smaller segments keep the grid full.
     if (N <= 4096) tw = 1;
     switch (tw) {
-    case 1: return launch_mfma<S, 1>(x, ldx, frags, K, B, N, y, ldy, stream);
-    case 4: return launch_mfma<S, 4>(x, ldx, frags, K, B, N, y, ldy, stream);
-    default: return launch_mfma<S, 2>(x, ldx, frags, K, B, N, y, ldy, stream);
+    case 1: return launch_mfma<S, 1>(x, ldx, frags, K, B, N, y, ldy, rev, stream);
+    case 4: return launch_mfma<S, 4>(x, ldx, frags, K, B, N, y, ldy, rev, stream);
+    default: return launch_mfma<S, 2>(x, ldx, frags, K, B, N, y, ldy, rev, stream);
     }
 }
 
 }  // namespace
 
 int fir_mfma_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
-                 float *y, int64_t ldy, int precision, hipStream_t stream, DeviceState *st)
+                 float *y, int64_t ldy, int precision, bool rev, hipStream_t stream,
+                 DeviceState *st)
 {
     if (B == 0 || N == 0) return SCIR_HIP_OK;
     if (K == 0) return fill_zero_2d(y, ldy, B, N, stream);
     if (precision == SCIR_HIP_PREC_BF16X3)
-        return run_mfma<3>(x, ldx, b, K, B, N, y, ldy, stream, st);
+        return run_mfma<3>(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
     if (precision == SCIR_HIP_PREC_FP16)
-        return run_mfma<1>(x, ldx, b, K, B, N, y, ldy, stream, st);
+        return run_mfma<1>(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
     return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "precision %d is not an MFMA mode", precision);
 }
 

@@ -1,0 +1,238 @@
+// Adjoint (vector-Jacobian product) kernels of the batched FIR.
+//
+// For L = sum_{r,n} g[r,n] * y[r,n], y = lfilter(b, [1], x) per row:
+//   dx[r,m] = sum_k b[k] * g[r,m+k]            (terms with m+k < N)
+//   db[k]   = sum_{r,n} g[r,n] * x[r,n-k]      (terms with n-k >= 0)
+//
+// dx is the forward filter run back to front (logical sample i == physical
+// N-1-i), so it reuses the dense direct/MFMA kernels through their `rev` mode
+// and inherits every precision mode.
+//
+// db is a correlation reduced over all rows and samples.  Stage 1: a
+// workgroup walks `tiles_per_wg` tiles of T = 2048 samples of one row,
+// staging g and x (with K samples of history) in LDS.  Lane (kg, tg) owns 16
+// lags k0 + kg*16 + (0..15) and 128 samples tg*128 + (0..127) of a tile; it
+// slides a 32-sample x window against 16 g values (256 FMAs per 12 LDS
+// reads), keeps a per-tile FP32 sum and adds it to a per-workgroup FP32
+// total.  The 16 time groups are reduced in a fixed order through LDS into
+// partial[wg][k].  Stage 2 sums the partials over workgroups with Kahan
+// compensation in a fixed order.  Nothing is atomic: results are
+// deterministic.  Lags beyond 256 are handled in passes of 256.
+#include "common.hpp"
+
+#include <algorithm>
+
+namespace scir {
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;
+constexpr int kTile = 2048;   // samples per tile
+constexpr int kLagPass = 256; // lags per pass (16 lag groups x 16 lags)
+
+__device__ __forceinline__ int pad16(int s) { return s + ((s >> 4) << 2); }
+
+__device__ __forceinline__ f32x4 load4g(const float *__restrict__ row, int64_t i, int64_t N,
+                                        bool vec)
+{
+    if (vec && i >= 0 && i + 3 < N) return *reinterpret_cast<const f32x4 *>(row + i);
+    f32x4 v;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = (i + c >= 0 && i + c < N) ? row[i + c] : 0.0f;
+    return v;
+}
+
+__global__ __launch_bounds__(kThreads) void fir_vjp_db_partial_kernel(
+    const float *__restrict__ g, int64_t ldg, const float *__restrict__ x, int64_t ldx, int K,
+    int64_t N, int segs_per_row, int tiles_per_wg, int hist, float *__restrict__ partial,
+    int vec_g, int vec_x)
+{
+    extern __shared__ float sm[];
+    float *gs = sm;                                 // kTile floats
+    float *xs = gs + kTile;                         // pad16(kTile + hist) floats
+    float *red = xs + pad16(kTile + hist + 32) + 4; // 16 x kLagPass floats
+
+    const int64_t row = blockIdx.x / segs_per_row;
+    const int64_t tile0 = (int64_t)(blockIdx.x % segs_per_row) * tiles_per_wg;
+    const float *__restrict__ grow = g + row * ldg;
+    const float *__restrict__ xrow = x + row * ldx;
+    const int kg = threadIdx.x & 15, tg = threadIdx.x >> 4;
+    const int64_t tiles_in_row = (N + kTile - 1) / kTile;
+    const int ntiles = (int)min<int64_t>(tiles_per_wg, tiles_in_row - tile0);
+
+    for (int k_pass = 0; k_pass < K; k_pass += kLagPass) {
+        const int k0 = k_pass + kg * 16;
+        float tot[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) tot[c] = 0.0f;
+        for (int it = 0; it < ntiles; ++it) {
+            const int64_t t0 = (tile0 + it) * kTile;
+            // hist >= k_pass + kLagPass + 16 samples of history before t0
+            const int64_t xbase = t0 - hist;
+            __syncthreads();  // previous tile done with gs/xs
+            for (int v = threadIdx.x; v < kTile / 4; v += kThreads)
+                *reinterpret_cast<f32x4 *>(gs + 4 * v) = load4g(grow, t0 + 4 * v, N, vec_g);
+            for (int v = threadIdx.x; v < (kTile + hist) / 4; v += kThreads)
+                *reinterpret_cast<f32x4 *>(xs + pad16(4 * v)) =
+                    load4g(xrow, xbase + 4 * v, N, vec_x);
+            __syncthreads();
+            float acc[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) acc[c] = 0.0f;
+            if (k0 < K) {
+#pragma unroll 2
+                for (int w = 0; w < 8; ++w) {
+                    const int n0 = tg * 128 + w * 16;
+                    float gv[16], xv[32];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const f32x4 a = *reinterpret_cast<const f32x4 *>(gs + n0 + 4 * j);
+                        gv[4 * j] = a.x; gv[4 * j + 1] = a.y; gv[4 * j + 2] = a.z; gv[4 * j + 3] = a.w;
+                    }
+                    // xv[j] = x[t0 + n0 - k0 - 16 + j] = xs[hist + n0 - k0 - 16 + j]
+                    const int xb = hist + n0 - k0 - 16;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const f32x4 a = *reinterpret_cast<const f32x4 *>(xs + pad16(xb + 4 * j));
+                        xv[4 * j] = a.x; xv[4 * j + 1] = a.y; xv[4 * j + 2] = a.z; xv[4 * j + 3] = a.w;
+                    }
+                    // acc[c] += sum_a g[n0+a] * x[n0+a-k0-c] = gv[a] * xv[a - c + 16]
+#pragma unroll
+                    for (int a = 0; a < 16; ++a)
+#pragma unroll
+                        for (int c = 0; c < 16; ++c)
+                            acc[c] = __builtin_fmaf(gv[a], xv[a - c + 16], acc[c]);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) tot[c] += acc[c];
+        }
+        // Fixed-order reduction over the 16 time groups.
+#pragma unroll
+        for (int c = 0; c < 16; ++c) red[tg * kLagPass + kg * 16 + c] = tot[c];
+        __syncthreads();
+        {
+            const int lag = k_pass + threadIdx.x;
+            float s = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) s += red[q * kLagPass + threadIdx.x];
+            if (lag < K) partial[(int64_t)blockIdx.x * K + lag] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// db[k] = sum_w partial[w][k], Kahan-compensated, fixed order: lane (lag, q)
+// sums workgroups w == q (mod 16), then the 16 sums are added in q order.
+__global__ __launch_bounds__(kThreads) void fir_vjp_db_final_kernel(
+    const float *__restrict__ partial, int64_t nwg, int K, float *__restrict__ db)
+{
+    __shared__ float sums[16][17];
+    const int lag_l = threadIdx.x & 15, q = threadIdx.x >> 4;
+    const int lag = blockIdx.x * 16 + lag_l;
+    float s = 0.0f, c = 0.0f;
+    if (lag < K) {
+        for (int64_t w = q; w < nwg; w += 16) {
+            const float y = partial[w * K + lag] - c;
+            const float t = s + y;
+            c = (t - s) - y;
+            s = t;
+        }
+    }
+    sums[q][lag_l] = s;
+    __syncthreads();
+    if (q == 0 && lag < K) {
+        float tot = 0.0f, cc = 0.0f;
+        for (int j = 0; j < 16; ++j) {
+            const float y = sums[j][lag_l] - cc;
+            const float t = tot + y;
+            cc = (t - tot) - y;
+            tot = t;
+        }
+        db[lag] = tot;
+    }
+}
+
+int aligned16(const float *p, int64_t ld)
+{
+    return ((reinterpret_cast<uintptr_t>(p) & 15) == 0) && (ld % 4 == 0);
+}
+
+}  // namespace
+}  // namespace scir
+
+using namespace scir;
+
+extern "C" {
+
+int scir_hip_fir_vjp_dx_dev(const float *g, int64_t ldg, const float *b, int32_t K, int64_t B,
+                            int64_t N, float *dx, int64_t lddx, int32_t precision, void *stream)
+{
+    clear_error();
+    SCIR_TRY(check_dense_args(g, ldg, b, K, B, N, dx, lddx));
+    if (B == 0 || N == 0) return SCIR_HIP_OK;
+    DeviceState *st = nullptr;
+    SCIR_TRY(current_device_state(&st));
+    std::lock_guard<std::mutex> lk(st->mu);
+    return fir_dense_dispatch(g, ldg, b, K, B, N, dx, lddx, precision, true, as_stream(stream), st);
+}
+
+int scir_hip_fir_vjp_db_dev(const float *g, int64_t ldg, const float *x, int64_t ldx, int32_t K,
+                            int64_t B, int64_t N, float *db, void *stream)
+{
+    clear_error();
+    if (K < 0 || B < 0 || N < 0)
+        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "negative size");
+    if (K == 0) return SCIR_HIP_OK;
+    if (!db) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "db is null");
+    if (B > 0 && N > 0) {
+        if (!g || !x) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "null pointer argument");
+        if (ldg < N || ldx < N)
+            return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row stride smaller than the row length");
+    }
+    DeviceState *st = nullptr;
+    SCIR_TRY(current_device_state(&st));
+    std::lock_guard<std::mutex> lk(st->mu);
+    hipStream_t s = as_stream(stream);
+    if (B == 0 || N == 0) {
+        // Empty sum: db = 0 (a one-row, zero-work pass of the final kernel).
+        hipLaunchKernelGGL(fir_vjp_db_final_kernel, dim3((K + 15) / 16), dim3(kThreads), 0, s,
+                           static_cast<const float *>(nullptr), (int64_t)0, (int)K, db);
+        return check_launch("fir_vjp_db_final_kernel");
+    }
+    const int64_t tiles_in_row = (N + kTile - 1) / kTile;
+    // Enough workgroups to fill the device (about 8 per CU), each walking a
+    // contiguous run of tiles of one row.
+    const int64_t want = (int64_t)st->cu_count * 8;
+    int64_t segs = std::max<int64_t>(1, std::min<int64_t>(tiles_in_row, (want + B - 1) / B));
+    const int64_t tiles_per_wg = (tiles_in_row + segs - 1) / segs;
+    segs = (tiles_in_row + tiles_per_wg - 1) / tiles_per_wg;
+    const int64_t nwg = B * segs;
+    if (nwg > 0x7fffffff) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "batch too large");
+    const int passes = (K + kLagPass - 1) / kLagPass;
+    const int hist = ((passes * kLagPass + 16) + 3) & ~3;  // history samples before a tile
+    const size_t lds_floats =
+        kTile + (size_t)(hist + kTile + 32) + ((hist + kTile + 32) / 16) * 4 + 4 + 16 * kLagPass;
+    const size_t lds = lds_floats * sizeof(float);
+    if (lds > 160 * 1024)
+        return fail(SCIR_HIP_ERR_UNSUPPORTED, "K=%d is too long for the dL/db kernel", K);
+    static bool attr_set = false;  // guarded by st->mu
+    if (!attr_set) {
+        SCIR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&fir_vjp_db_partial_kernel),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_set = true;
+    }
+    StreamScratch &sc = scratch_for(st, s);
+    SCIR_TRY(ensure_buffer(sc.partials, (size_t)nwg * K * sizeof(float), s));
+    float *partial = static_cast<float *>(sc.partials.ptr);
+    hipLaunchKernelGGL(fir_vjp_db_partial_kernel, dim3((unsigned)nwg), dim3(kThreads), lds, s, g,
+                       ldg, x, ldx, (int)K, N, (int)segs, (int)tiles_per_wg, hist, partial,
+                       aligned16(g, ldg), aligned16(x, ldx));
+    SCIR_TRY(check_launch("fir_vjp_db_partial_kernel"));
+    hipLaunchKernelGGL(fir_vjp_db_final_kernel, dim3((K + 15) / 16), dim3(kThreads), 0, s, partial,
+                       nwg, (int)K, db);
+    return check_launch("fir_vjp_db_final_kernel");
+}
+
+}  // extern "C"

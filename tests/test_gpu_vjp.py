@@ -1,0 +1,165 @@
+"""GPU parity of the adjoint kernels (dL/dx, dL/db) through the C ABI.
+
+L = sum(w * y), y = lfilter(b,[1],x) per row.  References: the FP64 oracle
+gradient (oracle.vjp_f64), central finite differences of the FP64 loss
+(exact up to rounding for a bilinear loss), and torch float64 autograd
+(through the test-only autograd wrapper).  Bounds:
+  dx: |dx - dx64| <= (K+1) * 2^-23 * sum_k |b_k||w_{m+k}|
+  db: |db - db64| <= (L + 200) * 2^-24 * sum_{r,n} |w_{r,n}||x_{r,n-k}|,
+      L = tiles per workgroup (<= samples/2048): the longest FP32 chain of
+      the two-stage reduction is 128 + L + 16 additions, the rest is Kahan.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rng
+
+pytestmark = pytest.mark.gpu
+
+EPS = 2.0 ** -23
+
+
+def _t(a):
+    import torch
+    return torch.tensor(np.ascontiguousarray(a), device="cuda")
+
+
+def gpu_grads(x, b, w, precision="f32"):
+    import torch
+    from scir_amd import device
+    xt, bt, wt = _t(x), _t(b), _t(w)
+    dx = device.vjp_dx(wt, bt, precision)
+    db = device.vjp_db(wt, xt, b.size)
+    torch.cuda.synchronize()
+    return dx.cpu().numpy(), db.cpu().numpy()
+
+
+def bounds(oracle, x, b, w):
+    dxa, dba = oracle.vjp_f64(np.abs(w), np.abs(x), np.abs(b))
+    N = x.shape[1]
+    K = b.size
+    return (K + 1) * EPS * dxa, (N / 2048 + 200) * 2.0 ** -24 * dba
+
+
+@pytest.mark.parametrize("B,N,K", [(1, 1, 1), (2, 5, 9), (3, 100, 1), (2, 2047, 33), (2, 2049, 255),
+                                   (1, 9000, 300), (3, 5000, 1024), (1, 7000, 3001), (4, 65536, 255)])
+@pytest.mark.parametrize("precision", ["f32", "bf16x3"])
+def test_grads_vs_fp64(hip, oracle, B, N, K, precision):
+    r = rng(B * 7 + N + K)
+    x = r.standard_normal((B, N)).astype(np.float32)
+    b = r.standard_normal(K).astype(np.float32)
+    w = r.standard_normal((B, N)).astype(np.float32)
+    dx, db = gpu_grads(x, b, w, precision)
+    dx64, db64 = oracle.vjp_f64(w, x, b)
+    bdx, bdb = bounds(oracle, x, b, w)
+    assert np.all(np.abs(dx - dx64) <= bdx), f"dx max excess {np.max(np.abs(dx - dx64) - bdx)}"
+    assert np.all(np.abs(db - db64) <= bdb), f"db max excess {np.max(np.abs(db - db64) - bdb)}"
+
+
+def test_finite_differences(hip, oracle):
+    """64 seeded coordinates (x and b), h = 1e-3, FP64 loss via the oracle."""
+    r = rng(64)
+    B, N, K = 3, 3000, 255
+    x = r.standard_normal((B, N)).astype(np.float32)
+    b = r.standard_normal(K).astype(np.float32)
+    w = r.standard_normal((B, N)).astype(np.float32)
+    dx, db = gpu_grads(x, b, w)
+    w64 = w.astype(np.float64)
+
+    def loss_rows(xx, bb):
+        return float(np.sum(w64 * oracle.fir_ref_f64(xx, bb[::-1].copy())))
+
+    h = 1e-3
+    x64 = x.astype(np.float64)
+    b64 = b.astype(np.float64)
+    for _ in range(48):
+        row, m = int(r.integers(B)), int(r.integers(N))
+        xp = x64.copy(); xm = x64.copy()
+        xp[row, m] += h; xm[row, m] -= h
+        fd = (loss_rows(xp[row:row + 1], b64) - loss_rows(xm[row:row + 1], b64)) / (2 * h)
+        # the loss of the other rows cancels; only row `row` is evaluated
+        fd = fd if True else 0.0
+        assert abs(fd - dx[row, m]) <= 1e-5 * (1.0 + abs(fd)), (row, m, fd, dx[row, m])
+    for _ in range(16):
+        k = int(r.integers(K))
+        bp = b64.copy(); bm = b64.copy()
+        bp[k] += h; bm[k] -= h
+        fd = (loss_rows(x64, bp) - loss_rows(x64, bm)) / (2 * h)
+        assert abs(fd - db[k]) <= 1e-5 * (1.0 + abs(fd)), (k, fd, db[k])
+
+
+def test_autograd_wrapper_vs_torch_f64(hip):
+    import torch
+    from scir_amd import autograd
+    r = rng(3)
+    B, N, K = 2, 4000, 129
+    x = r.standard_normal((B, N)).astype(np.float32)
+    b = r.standard_normal(K).astype(np.float32)
+    w = r.standard_normal((B, N)).astype(np.float32)
+    xt = torch.tensor(x, device="cuda", requires_grad=True)
+    bt = torch.tensor(b, device="cuda", requires_grad=True)
+    y = autograd.fir(xt, bt, "f32")
+    (y * torch.tensor(w, device="cuda")).sum().backward()
+    xc = torch.tensor(x, dtype=torch.float64, requires_grad=True)
+    bc = torch.tensor(b, dtype=torch.float64, requires_grad=True)
+    yc = torch.nn.functional.conv1d(torch.nn.functional.pad(xc[:, None, :], (K - 1, 0)),
+                                    bc.flip(0)[None, None, :])[:, 0, :]
+    np.testing.assert_allclose(y.detach().cpu().numpy(), yc.detach().numpy(), rtol=1e-4, atol=1e-4)
+    (yc * torch.tensor(w, dtype=torch.float64)).sum().backward()
+    np.testing.assert_allclose(xt.grad.cpu().numpy(), xc.grad.numpy(), rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(bt.grad.cpu().numpy(), bc.grad.numpy(), rtol=1e-4, atol=1e-3)
+
+
+def test_empty_and_strided(hip, oracle):
+    import torch
+    from scir_amd import device
+    r = rng(4)
+    K = 17
+    b = torch.tensor(r.standard_normal(K).astype(np.float32), device="cuda")
+    x0 = torch.zeros((0, 10), device="cuda")
+    db = device.vjp_db(x0, x0, K)
+    torch.cuda.synchronize()
+    assert not db.cpu().numpy().any()
+    base_g = torch.tensor(r.standard_normal((3, 1003)).astype(np.float32), device="cuda")
+    base_x = torch.tensor(r.standard_normal((3, 1005)).astype(np.float32), device="cuda")
+    g, x = base_g[:, 1:1000], base_x[:, 3:1002]
+    dx = device.vjp_dx(g, b)
+    dbv = device.vjp_db(g, x, K)
+    torch.cuda.synchronize()
+    dx64, db64 = oracle.vjp_f64(g.cpu().numpy(), x.cpu().numpy(), b.cpu().numpy())
+    bdx, bdb = bounds(oracle, x.cpu().numpy(), b.cpu().numpy(), g.cpu().numpy())
+    assert np.all(np.abs(dx.cpu().numpy() - dx64) <= bdx)
+    assert np.all(np.abs(dbv.cpu().numpy() - db64) <= bdb)
+
+
+def test_full_size_differentiation_workload(hip, oracle):
+    """x [1024, 65536], 255-tap low-pass, w ~ N(0,1): dx on a row subset vs
+    FP64; db over all rows checked by additivity over row halves and on a
+    row subset vs FP64."""
+    import torch
+    from scir_amd import design, device
+    B, N, K = 1024, 65536, 255
+    print(f"full-size vjp B={B} N={N} K={K}", flush=True)
+    x = device.synth_normal(B, N, seed=0x5C120004)
+    w = device.synth_normal(B, N, seed=0x5C120005)
+    b_np = design.lowpass(K)
+    b = torch.tensor(b_np, device="cuda")
+    dx = device.vjp_dx(w, b, "bf16x3")
+    db_all = device.vjp_db(w, x, K)
+    db_lo = device.vjp_db(w[:512], x[:512], K)
+    db_hi = device.vjp_db(w[512:], x[512:], K)
+    rows = [0, 511, 1023]
+    db_sub = device.vjp_db(w[rows], x[rows], K)
+    torch.cuda.synchronize()
+    xs, ws = x[rows].cpu().numpy(), w[rows].cpu().numpy()
+    dx64, db64 = oracle.vjp_f64(ws, xs, b_np)
+    bdx, bdb = bounds(oracle, xs, b_np, ws)
+    assert np.all(np.abs(dx[rows].cpu().numpy() - dx64) <= bdx)
+    assert np.all(np.abs(db_sub.cpu().numpy() - db64) <= bdb)
+    # additivity over rows; scale from |w||x| of the full problem
+    sabs = device.vjp_db(w.abs(), x.abs(), K)
+    torch.cuda.synchronize()
+    lim = (N / 2048 + 200) * 2.0 ** -24 * 2.0 * sabs.cpu().numpy()
+    assert np.all(np.abs(db_all.cpu().numpy() - (db_lo + db_hi).cpu().numpy()) <= lim)
