@@ -68,8 +68,8 @@ def test_finite_differences(hip, oracle):
     dx, db = gpu_grads(x, b, w)
     w64 = w.astype(np.float64)
 
-    def loss_rows(xx, bb):
-        return float(np.sum(w64 * oracle.fir_ref_f64(xx, bb[::-1].copy())))
+    def loss_rows(xx, bb, rows=slice(None)):
+        return float(np.sum(w64[rows] * oracle.fir_ref_f64(xx, bb[::-1].copy())))
 
     h = 1e-3
     x64 = x.astype(np.float64)
@@ -78,9 +78,9 @@ def test_finite_differences(hip, oracle):
         row, m = int(r.integers(B)), int(r.integers(N))
         xp = x64.copy(); xm = x64.copy()
         xp[row, m] += h; xm[row, m] -= h
-        fd = (loss_rows(xp[row:row + 1], b64) - loss_rows(xm[row:row + 1], b64)) / (2 * h)
-        # the loss of the other rows cancels; only row `row` is evaluated
-        fd = fd if True else 0.0
+        # only row `row` of the loss depends on x[row, m]
+        one = slice(row, row + 1)
+        fd = (loss_rows(xp[one], b64, one) - loss_rows(xm[one], b64, one)) / (2 * h)
         assert abs(fd - dx[row, m]) <= 1e-5 * (1.0 + abs(fd)), (row, m, fd, dx[row, m])
     for _ in range(16):
         k = int(r.integers(K))
