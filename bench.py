@@ -1,17 +1,27 @@
 """Benchmark of the batched-FIR hot path on one MI355X.
 
-Prints ONE JSON line (rank 0):
-  metric/value/unit   Gsamples/s of the headline workload (output samples per
-                      second, inputs resident in HBM, device time of K steps)
-  roofline            dominant kernel: algorithmic bytes or flops per launch /
-                      its average launch time (HIP events on the launch stream)
-  cpu_baseline        the oracle's restatement of the reference CPU loop
-                      (single thread) on a bounded sample of the same workload
+Prints ONE JSON line.  The headline (``value``) is the long-tap workload in
+BF16x3 on MFMA:
 
-A step = one pass of the FIR over the whole synthetic batch.  Workloads:
-  direct_k33   x[4096, 65536] N(0,1), 33-tap Hamming low-pass 0.2, FP32 direct
-  direct_k129  same x, 129-tap low-pass, FP32 direct
-(``--workload`` selects; default is the headline workload.)
+  longtap_k512   x [2048, 262144] f32 ~ N(0,1), 1% of samples scaled x1e3,
+                 512-tap Kaiser(beta=8) band-pass 0.1-0.3 of Nyquist
+
+A step is one pass of the FIR over the whole batch, inputs already resident
+in HBM.  ``value`` = output samples / second over the K timed steps (device
+time, HIP events on the launch stream).  Beside it the line carries, measured
+after the timed region on the same inputs:
+  compare.fp32_direct / compare.fp16  the other arithmetic paths, Gsamples/s
+                                      and rel-L2 vs the FP64 oracle
+  bandwidth_workload                  x [4096, 65536], 33-tap low-pass, FP32
+                                      direct: GB/s against the 8 TB/s HBM peak
+  roofline                            the dominant kernel of the headline
+  cpu_baseline                        the oracle's restatement of the
+                                      reference CPU loop, one thread, on whole
+                                      rows of the same workload (~10 s)
+
+Other workloads (``--workload``): longtap_k2048, direct_k33, direct_k129,
+ragged (10000 CSR signals), vjp_k255 (forward + dL/dx + dL/db), copy_1gib
+(plain device copy: attainable bandwidth and HBM-counter calibration).
 """
 from __future__ import annotations
 
@@ -28,126 +38,270 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0   # dense bf16/fp16 MFMA, MI355X
 FP32_PEAK_TFLOPS = 157.3
-BF16_PEAK_TFLOPS = 2500.0
 
-WORKLOADS = {
+METRIC = "FIR Gsamples/s & %roofline: bf16x3 MFMA vs FP32 direct at rel L2 err<=1e-6"
+
+DENSE = {
+    "longtap_k512": dict(B=2048, N=262144, K=512, taps="bandpass", precision="bf16x3",
+                         seed=0x5C120002, outliers=0.01),
+    "longtap_k2048": dict(B=2048, N=262144, K=2048, taps="bandpass", precision="bf16x3",
+                          seed=0x5C120002, outliers=0.01),
     "direct_k33": dict(B=4096, N=65536, K=33, taps="lowpass", precision="f32",
-                       bound="hbm", seed=0x5C120001, outliers=0.0),
+                       seed=0x5C120001, outliers=0.0),
     "direct_k129": dict(B=4096, N=65536, K=129, taps="lowpass", precision="f32",
-                        bound="hbm", seed=0x5C120001, outliers=0.0),
+                        seed=0x5C120001, outliers=0.0),
 }
-HEADLINE = "direct_k33"
+HEADLINE = "longtap_k512"
+
+
+def mfma_steps(K):
+    return (K + 14) // 16 + 2
 
 
 def make_taps(kind, K):
     from scir_amd import design
     if kind == "lowpass":
         return design.lowpass(K, 0.2)
-    if kind == "bandpass":
-        return design.bandpass_kaiser(K, (0.1, 0.3), 8.0)
-    raise ValueError(kind)
+    return design.bandpass_kaiser(K, (0.1, 0.3), 8.0)
 
 
-def load_traffic(kernel_tag):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
-    path = os.path.join(ROOT, "profiles", f"{kernel_tag}_pmc.json")
+def describe_taps(kind, K):
+    if kind == "lowpass":
+        return f"{K}-tap Hamming low-pass 0.2 of Nyquist, unit DC gain"
+    return f"{K}-tap Kaiser(8) band-pass 0.1-0.3 of Nyquist"
+
+
+def load_traffic(tag):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", f"{tag}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        d = json.load(f)
-    return d.get("hbm_bytes_per_launch")
+        return json.load(f).get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(x_rows, b, budget_s=10.0):
+def timed(fn, steps, warmup, stream):
+    """Run fn warmup times, then time `steps` calls with HIP events on
+    `stream`.  Returns (ms_per_step, per-call ms list, wall seconds)."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    t_wall = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        fn()
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall
+    per = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    return starts[0].elapsed_time(ends[-1]) / steps, per, wall
+
+
+def cpu_baseline(rows_np, b, budget_s):
     """Single-thread oracle (restatement of crates/scir-gpu/src/lib.rs:1134-1152)
     over whole rows of the workload until ~budget_s of CPU time is spent."""
     from oracle import oracle as orc
     taps = np.ascontiguousarray(b[::-1])
     done = 0
     t0 = time.perf_counter()
-    while done < x_rows.shape[0]:
-        orc.fir_ref_f32(x_rows[done:done + 1], taps)
+    while done < rows_np.shape[0]:
+        orc.fir_ref_f32(rows_np[done:done + 1], taps)
         done += 1
         if time.perf_counter() - t0 >= budget_s:
             break
     dt = time.perf_counter() - t0
-    n = x_rows.shape[1]
-    return dict(value=done * n / dt / 1e9, unit="Gsamples/s", cores=1, kind="port",
-                sample=f"{done} full rows x {n} samples of the same workload, 1 thread, "
-                       f"{dt:.1f} s")
+    n = rows_np.shape[1]
+    return dict(value=round(done * n / dt / 1e9, 6), unit="Gsamples/s", cores=1, kind="port",
+                sample=f"{done} whole rows x {n} samples of the same workload, "
+                       f"1 thread, {dt:.1f} s")
 
 
-def run(args):
+def dense_setup(name):
     import torch
-    from scir_amd import _lib, device
-    from oracle import oracle as orc
-
-    wl = WORKLOADS[args.workload]
-    B, N, K = wl["B"], wl["N"], wl["K"]
-    torch.cuda.set_device(0)
-    _lib.check(_lib.lib().scir_hip_init())
-    stream = torch.cuda.current_stream()
-    x = device.synth_normal(B, N, seed=wl["seed"], outlier_frac=wl["outliers"],
+    from scir_amd import device
+    wl = DENSE[name]
+    x = device.synth_normal(wl["B"], wl["N"], seed=wl["seed"], outlier_frac=wl["outliers"],
                             outlier_scale=1000.0)
-    b_np = make_taps(wl["taps"], K)
-    b = torch.tensor(b_np, device="cuda")
-    y = torch.empty_like(x)
-    prec = args.precision or wl["precision"]
+    b_np = make_taps(wl["taps"], wl["K"])
+    return wl, x, torch.tensor(b_np, device="cuda"), b_np
 
-    for _ in range(args.warmup):
-        device.fir(x, b, prec, out=y)
-    torch.cuda.synchronize()
 
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    t_wall = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record(stream)
-        device.fir(x, b, prec, out=y)
-        ends[i].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_wall
-    per_launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-    total_ms = starts[0].elapsed_time(ends[-1])
-    ms_per_step = total_ms / args.steps
-    samples = B * N
-    value = samples / (ms_per_step * 1e-3) / 1e9
-    avg_launch_s = float(np.mean(per_launch_ms)) * 1e-3
-
-    # accuracy on a row subset vs the FP64 oracle
-    rows = [0, B // 3, B - 1]
+def rel_l2_rows(x, y, b_np, rows):
+    from oracle import oracle as orc
     xs = x[rows].cpu().numpy()
-    y64 = orc.lfilter_f64(xs, b_np)
-    rel = orc.rel_l2(y[rows].cpu().numpy(), y64)
+    return orc.rel_l2(y[rows].cpu().numpy(), orc.lfilter_f64(xs, b_np))
 
-    if wl["bound"] == "hbm":
-        alg = 8.0 * samples  # 4 B read + 4 B written per output sample
-        achieved = alg / avg_launch_s / 1e9
-        roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4),
-                    traffic=load_traffic(f"fir_direct_{args.workload}"))
-    else:
-        alg = 2.0 * K * samples
-        achieved = alg / avg_launch_s / 1e12
-        roof = dict(bound="mfma", achieved=round(achieved, 2), peak=BF16_PEAK_TFLOPS,
-                    unit="TFLOP/s", frac=round(achieved / BF16_PEAK_TFLOPS, 4),
-                    traffic=load_traffic(f"fir_mfma_{args.workload}"))
 
-    out = dict(
-        metric="FIR Gsamples/s & %roofline: bf16x3 MFMA vs FP32 direct at rel L2 err<=1e-6",
-        value=round(value, 2), unit="Gsamples/s", n_gpus=1, steps=args.steps,
-        warmup=args.warmup, ms_per_step=round(ms_per_step, 4), higher_is_better=True,
-        scaling="weak", vs_baseline=None, dtype="f32", data="synthetic",
-        config=dict(workload=args.workload, rows=B, samples_per_row=N, taps=K,
-                    taps_design=wl["taps"], precision=prec),
-        rel_l2_vs_fp64=rel, wall_s=round(wall, 3), roofline=roof,
-    )
+def roofline_for(wl, name, prec, avg_launch_s):
+    samples = wl["B"] * wl["N"]
+    if prec in ("bf16x3", "fp16"):
+        alg = 2.0 * wl["K"] * samples
+        ach = alg / avg_launch_s / 1e12
+        prods = 6 if prec == "bf16x3" else 1
+        issued = 32.0 * mfma_steps(wl["K"]) * prods * samples
+        return dict(bound="mfma", achieved=round(ach, 2), peak=BF16_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=round(ach / BF16_PEAK_TFLOPS, 4),
+                    traffic=load_traffic(f"fir_mfma_{prec}_{name}_pmc"),
+                    mfma_issued_tflops=round(issued / avg_launch_s / 1e12, 1),
+                    mfma_issued_frac=round(issued / avg_launch_s / 1e12 / BF16_PEAK_TFLOPS, 4))
+    flops = 2.0 * wl["K"] * samples
+    if flops / (8.0 * samples) < 78.6 / 8.0:  # below the FP32-vector ridge: HBM-bound
+        alg = 8.0 * samples
+        ach = alg / avg_launch_s / 1e9
+        return dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4),
+                    traffic=load_traffic(f"fir_direct_{prec}_{name}_pmc"))
+    ach = flops / avg_launch_s / 1e12
+    return dict(bound="fp32", achieved=round(ach, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                frac=round(ach / FP32_PEAK_TFLOPS, 4),
+                traffic=load_traffic(f"fir_direct_{prec}_{name}_pmc"))
+
+
+def run_dense(args):
+    import torch
+    from scir_amd import device
+    name = args.workload
+    wl, x, b, b_np = dense_setup(name)
+    prec = args.precision or wl["precision"]
+    B, N, K = wl["B"], wl["N"], wl["K"]
+    y = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    ms, per, wall = timed(lambda: device.fir(x, b, prec, out=y), args.steps, args.warmup, stream)
+    value = B * N / (ms * 1e-3) / 1e9
+    rows = [0, B // 2, B - 1]
+    out = dict(metric=METRIC, value=round(value, 2), unit="Gsamples/s", n_gpus=1,
+               steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 4),
+               higher_is_better=True, scaling="weak", vs_baseline=None, dtype=prec,
+               data="synthetic",
+               config=dict(workload=name, rows=B, samples_per_row=N, taps=K,
+                           taps_design=describe_taps(wl["taps"], K),
+                           outliers="1% of samples x1e3" if wl["outliers"] else "none",
+                           precision=prec),
+               rel_l2_vs_fp64=rel_l2_rows(x, y, b_np, rows), wall_s=round(wall, 3),
+               roofline=roofline_for(wl, name, prec, float(np.mean(per)) * 1e-3))
+    if not args.quick:
+        comp = {}
+        for other in ("f32", "fp16", "bf16x3"):
+            if other == prec:
+                continue
+            ms_o, _, _ = timed(lambda: device.fir(x, b, other, out=y), max(3, args.steps // 4),
+                               1, stream)
+            comp["fp32_direct" if other == "f32" else other] = dict(
+                gsamples_s=round(B * N / (ms_o * 1e-3) / 1e9, 2), ms_per_step=round(ms_o, 4),
+                rel_l2_vs_fp64=rel_l2_rows(x, y, b_np, rows))
+        out["compare"] = comp
+        if name != "direct_k33":
+            wl2, x2, b2, _ = dense_setup("direct_k33")
+            y2 = torch.empty_like(x2)
+            ms2, per2, _ = timed(lambda: device.fir(x2, b2, "f32", out=y2), 10, 3, stream)
+            r2 = roofline_for(wl2, "direct_k33", "f32", float(np.mean(per2)) * 1e-3)
+            out["bandwidth_workload"] = dict(
+                workload="direct_k33", rows=wl2["B"], samples_per_row=wl2["N"], taps=33,
+                precision="f32", gsamples_s=round(wl2["B"] * wl2["N"] / (ms2 * 1e-3) / 1e9, 2),
+                ms_per_step=round(ms2, 4), roofline=r2)
+            del x2, y2
     if not args.no_cpu_baseline:
-        nrows = min(B, 512)
-        out["cpu_baseline"] = cpu_baseline(x[:nrows].cpu().numpy(), b_np,
-                                           budget_s=args.cpu_budget)
-    print(json.dumps(out), flush=True)
+        nrows = min(B, 256)
+        out["cpu_baseline"] = cpu_baseline(x[:nrows].cpu().numpy(), b_np, args.cpu_budget)
+    return out
+
+
+def run_ragged(args):
+    import torch
+    from scir_amd import configs, device
+    from scir_amd.ragged import Graph, RaggedPlan
+    offs, boffs, taps = configs.ragged_layout()
+    total = int(offs[-1])
+    x = device.synth_normal(1, total, seed=configs.SEED_BASE + 3).view(-1)
+    bt = torch.tensor(taps, device="cuda")
+    y = torch.empty_like(x)
+    plan = RaggedPlan(offs, boffs, "f32")
+    stream = torch.cuda.current_stream()
+    ms, per, wall = timed(lambda: plan.execute(x, bt, y), args.steps, args.warmup, stream)
+    macs = float(np.sum(np.diff(offs) * np.diff(boffs)))
+    out = dict(metric=METRIC, value=round(total / (ms * 1e-3) / 1e9, 2), unit="Gsamples/s",
+               n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 4),
+               higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
+               data="synthetic",
+               config=dict(workload="ragged", signals=offs.size - 1, samples=total,
+                           lengths="log-uniform [256, 2^20]", taps="uniform [15, 1023]",
+                           precision="f32"),
+               wall_s=round(wall, 3))
+    ach = 2.0 * macs / (float(np.mean(per)) * 1e-3) / 1e12
+    out["roofline"] = dict(bound="fp32", achieved=round(ach, 2), peak=FP32_PEAK_TFLOPS,
+                           unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 4),
+                           traffic=load_traffic("fir_ragged_f32_ragged_pmc"))
+    ms_s, _, _ = timed(lambda: plan.execute_per_signal(x, bt, y), 1, 1, stream)
+    side = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with Graph(side) as g:
+        plan.execute(x, bt, y)
+
+    def replay():
+        g.launch()
+    ms_g, _, _ = timed(replay, max(3, args.steps // 2), 1, side)
+    out["compare"] = dict(per_signal_launches_ms=round(ms_s, 3), graph_replay_ms=round(ms_g, 3),
+                          persistent_speedup=round(ms_s / ms, 2))
+    g.close()
+    plan.close()
+    return out
+
+
+def run_vjp(args):
+    import torch
+    from scir_amd import design, device
+    B, N, K = 1024, 65536, 255
+    x = device.synth_normal(B, N, seed=0x5C120004)
+    w = device.synth_normal(B, N, seed=0x5C120005)
+    b = torch.tensor(design.lowpass(K), device="cuda")
+    y, dx = torch.empty_like(x), torch.empty_like(x)
+    db = torch.empty(K, device="cuda")
+    prec = args.precision or "bf16x3"
+
+    def step():
+        device.fir(x, b, prec, out=y)
+        device.vjp_dx(w, b, prec, out=dx)
+        device.vjp_db(w, x, K, out=db)
+    stream = torch.cuda.current_stream()
+    ms, per, wall = timed(step, args.steps, args.warmup, stream)
+    ms_db, per_db, _ = timed(lambda: device.vjp_db(w, x, K, out=db), 5, 1, stream)
+    flops_db = 2.0 * K * B * N
+    ach = flops_db / (float(np.mean(per_db)) * 1e-3) / 1e12
+    return dict(metric=METRIC, value=round(B * N / (ms * 1e-3) / 1e9, 2), unit="Gsamples/s",
+                n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 4),
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype=prec,
+                data="synthetic",
+                config=dict(workload="vjp_k255", rows=B, samples_per_row=N, taps=K,
+                            step="forward + dL/dx + dL/db", precision=prec),
+                wall_s=round(wall, 3),
+                roofline=dict(bound="fp32", achieved=round(ach, 2), peak=FP32_PEAK_TFLOPS,
+                              unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 4),
+                              kernel="fir_vjp_db_partial_kernel", traffic=None))
+
+
+def run_copy(args):
+    """Plain device copy of 1 GiB (2^28 floats): the attainable-bandwidth
+    reference and the calibration kernel for the HBM byte counters."""
+    import torch
+    from scir_amd import device
+    n = 1 << 28
+    x = device.synth_normal(1, n, seed=1).view(-1)
+    y = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    ms, per, wall = timed(lambda: y.copy_(x), args.steps, args.warmup, stream)
+    gbs = 8.0 * n / (float(np.mean(per)) * 1e-3) / 1e9
+    return dict(metric="device copy GB/s (calibration)", value=round(gbs, 1), unit="GB/s",
+                n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 4),
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
+                data="synthetic", config=dict(workload="copy_1gib", bytes_read=4 * n,
+                                              bytes_written=4 * n),
+                roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS,
+                              unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None))
 
 
 def main():
@@ -155,18 +309,32 @@ def main():
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default=HEADLINE, choices=sorted(WORKLOADS))
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default=HEADLINE,
+                    choices=sorted(DENSE) + ["ragged", "vjp_k255", "copy_1gib"])
     ap.add_argument("--precision", default=None)
+    ap.add_argument("--quick", action="store_true", help="skip the side comparisons")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
-              "rows shard across GPUs with no exchange, but only the one-GPU figure is "
-              "measured here.", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested). "
+              "Rows and ragged signals shard across GPUs with no exchange; only the one-GPU "
+              "figure is measured here.", file=sys.stderr)
         sys.exit(2)
-    run(args)
+    import torch
+    from scir_amd import _lib
+    torch.cuda.set_device(0)
+    _lib.check(_lib.lib().scir_hip_init())
+    if args.workload == "ragged":
+        out = run_ragged(args)
+    elif args.workload == "vjp_k255":
+        out = run_vjp(args)
+    elif args.workload == "copy_1gib":
+        out = run_copy(args)
+    else:
+        out = run_dense(args)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_kernel(
     float *__restrict__ y, int64_t ldy, int64_t row0, int tiles_per_row, int vec_x, int vec_y,
     int rev)
 {
-    extern __shared__ float xs[];
+    extern __shared__ __attribute__((aligned(16))) float xs[];
     constexpr int T = kThreads * R;
     const int64_t row = row0 + blockIdx.x / tiles_per_row;
     const int64_t t0 = (int64_t)(blockIdx.x % tiles_per_row) * T;
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_pipe_kernel(
     float *__restrict__ y, int64_t ldy, int64_t row0, int segs_per_row, int tiles_per_seg,
     int64_t tiles_per_row, int buf_floats, int vec_x, int vec_y, int rev)
 {
-    extern __shared__ float lds[];
+    extern __shared__ __attribute__((aligned(16))) float lds[];
     constexpr int T = kThreads * R;
     const int H = (K + 6) & ~3;
     const int count4 = (T + H + 4) / 4;  // <= NS * kThreads (checked by the launcher)

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kThreads) void fir_ragged_kernel(
     const float *__restrict__ x, const float *__restrict__ b, float *__restrict__ y,
     const RaggedItem *__restrict__ items, int n_items, unsigned int *__restrict__ counter)
 {
-    extern __shared__ float xs[];
+    extern __shared__ __attribute__((aligned(16))) float xs[];
     __shared__ int s_item;
     const int lane_base = threadIdx.x * kR;
     for (;;) {

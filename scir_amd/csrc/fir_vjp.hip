@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kThreads) void fir_vjp_db_partial_kernel(
     int64_t N, int segs_per_row, int tiles_per_wg, int hist, float *__restrict__ partial,
     int vec_g, int vec_x)
 {
-    extern __shared__ float sm[];
+    extern __shared__ __attribute__((aligned(16))) float sm[];
     float *gs = sm;                                 // kTile floats
     float *xs = gs + kTile;                         // pad16(kTile + hist) floats
     float *red = xs + pad16(kTile + hist + 32) + 4; // 16 x kLagPass floats
