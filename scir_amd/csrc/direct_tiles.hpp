@@ -152,10 +152,10 @@ __device__ __forceinline__ void store_outputs(const float (&acc)[R], float *__re
         for (int r = 0; r < R; r += 4) {
             if (!rev) {
                 f32x4 v = {acc[r], acc[r + 1], acc[r + 2], acc[r + 3]};
-                __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(yrow + i0 + r));
+                *reinterpret_cast<f32x4 *>(yrow + i0 + r) = v;
             } else {
                 f32x4 v = {acc[r + 3], acc[r + 2], acc[r + 1], acc[r]};
-                __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(yrow + (N - 4 - i0 - r)));
+                *reinterpret_cast<f32x4 *>(yrow + (N - 4 - i0 - r)) = v;
             }
         }
     } else {

@@ -326,10 +326,9 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
             const int64_t n = n0 + 8 * q;
             if (vec_y && n + 3 < N) {
                 if (!rev)
-                    __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(yrow + n));
+                    *reinterpret_cast<f32x4 *>(yrow + n) = v;
                 else
-                    __builtin_nontemporal_store(f32x4{v.w, v.z, v.y, v.x},
-                                                reinterpret_cast<f32x4 *>(yrow + (N - 4 - n)));
+                    *reinterpret_cast<f32x4 *>(yrow + (N - 4 - n)) = f32x4{v.w, v.z, v.y, v.x};
             } else {
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
