@@ -25,8 +25,9 @@
  *     void* (NULL = the null stream); they enqueue work and return without
  *     synchronising.  Callers own all memory they pass in.
  *   - Thread safety: every entry point may be called from any thread.  Work
- *     that shares library-owned scratch (the MFMA tap fragments, the dL/db
- *     partial sums, the host-tier staging buffers) is serialised per device.
+ *     that shares library-owned scratch (the dL/db partial sums, the
+ *     host-tier staging buffers) is serialised per device; the dL/db partials
+ *     are kept per stream.
  */
 #ifndef SCIR_HIP_H
 #define SCIR_HIP_H

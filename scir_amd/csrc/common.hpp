@@ -46,8 +46,7 @@ struct DeviceBuffer {
 // are ordered by the stream, so they may reuse the buffers; calls on other
 // streams get their own set.
 struct StreamScratch {
-    DeviceBuffer tap_frags;   // MFMA A-operand fragments of the current taps
-    DeviceBuffer partials;    // per-tile dL/db partial sums
+    DeviceBuffer partials;    // per-workgroup dL/db partial sums
 };
 
 struct DeviceState {

@@ -8,8 +8,13 @@
 //   B_m[c][p] = x[32p - 16m + c]          (16 x 32 block columns)
 //   y[32p + r] = sum_m (A_m B_m)[r][p]
 // Each (k, output) pair appears exactly once (m = ceil((k-r)/16), c = 16m-k+r).
-// A_m depends only on the taps: it is shared by every row and tile and is
-// prepared once per call (fir_mfma_frags_kernel) as per-lane MFMA fragments.
+// A_m depends only on the taps.  Each workgroup keeps the split taps in LDS
+// as a compact reversed table, T[j] = b[C - j]: lane (i, g) of A_m holds the
+// 8 consecutive entries starting at j0 = C - 16m - i + 8g, i.e. one 16-byte
+// read per split and half-step.  The table is stored twice, the second copy
+// shifted by one entry, so every lane's read is 4-byte aligned whatever the
+// parity of j0; the two copies are placed 128 bytes apart modulo the 256-byte
+// bank cycle so the lanes reading either copy do not collide.
 // B_m is a shifted view of the same staged x: no im2col copy is made; each
 // sample is split once into LDS and read by the ~2*(K+31)/32 half-steps that
 // need it.
@@ -47,10 +52,6 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
 constexpr int kMaxLds = 160 * 1024;
-// The main loop reads tap fragments of half-steps mi+2 and mi+3 for even mi <
-// msteps, i.e. up to index msteps+2: the fragment buffer carries this many
-// zero-filled steps past the last real one.
-constexpr int kLookahead = 4;
 constexpr int kStageBatch = 4;  // LDS units (8 samples each) loaded per lane before splitting
 
 inline int mfma_m_hi(int K) { return (K + 14) / 16; }  // ceil((K-1)/16) for K >= 1
@@ -79,35 +80,6 @@ __device__ __forceinline__ void split3(float v, unsigned short &h, unsigned shor
 __device__ __forceinline__ unsigned int pack2(unsigned short lo, unsigned short hi)
 {
     return (unsigned int)lo | ((unsigned int)hi << 16);
-}
-
-// Tap fragments: frags[(mi*S + s)*64 + lane] holds the 8 bf16/fp16 values of
-// A_m (m = mi - 1) for `lane`, split term s.
-template <int S>
-__global__ void fir_mfma_frags_kernel(const float *__restrict__ b, int K, int msteps,
-                                      u32x4 *__restrict__ frags)
-{
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= msteps * 64) return;
-    const int lane = idx & 63, mi = idx >> 6, m = mi - 1;
-    const int r = lane & 31, g = lane >> 5;
-    unsigned short t[S][8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const int k = 16 * m + r - 8 * g - e;
-        const float v = (k >= 0 && k < K) ? b[k] : 0.0f;
-        if (S == 3) {
-            split3(v, t[0][e], t[S > 1 ? 1 : 0][e], t[S > 2 ? 2 : 0][e]);
-        } else {
-            t[0][e] = f16_bits((_Float16)v);
-        }
-    }
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        u32x4 w = {pack2(t[s][0], t[s][1]), pack2(t[s][2], t[s][3]), pack2(t[s][4], t[s][5]),
-                   pack2(t[s][6], t[s][7])};
-        frags[(mi * S + s) * 64 + lane] = w;
-    }
 }
 
 template <int S>
@@ -141,14 +113,24 @@ __device__ __forceinline__ void step_mfma(f32x16 (&acc0)[TW], f32x16 (&acc1)[TW]
     }
 }
 
-// Tap fragments of half-steps mi and mi+1.
+// 16 bytes from a 4-byte aligned LDS address.
+__device__ __forceinline__ u32x4 lds_read16_a4(const unsigned char *p)
+{
+    const unsigned int *q = reinterpret_cast<const unsigned int *>(p);
+    return u32x4{q[0], q[1], q[2], q[3]};
+}
+
+// A operands of half-steps mi and mi+1: `abase[s]` is this lane's byte
+// address of half-step 0 in split s's table; each half-step moves 16 entries
+// (32 bytes) down the reversed table.
 template <int S>
-__device__ __forceinline__ void load_a(u32x4 (&a)[2][S], const u32x4 *__restrict__ fr, int mi)
+__device__ __forceinline__ void load_a(u32x4 (&a)[2][S], const unsigned char *lds,
+                                       const int (&abase)[S], int mi)
 {
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int s = 0; s < S; ++s) a[h][s] = fr[((mi + h) * S + s) * 64];
+        for (int s = 0; s < S; ++s) a[h][s] = lds_read16_a4(lds + abase[s] - 32 * (mi + h));
 }
 
 // B operands of half-step mi for TW tiles (unit clamped to 0 for look-ahead
@@ -179,8 +161,9 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb)
 template <int S, int TW>
 __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
     const float *__restrict__ x, int64_t ldx, int64_t N, float *__restrict__ y, int64_t ldy,
-    const u32x4 *__restrict__ frags, int msteps, int m_hi, int64_t row0, int segs_per_row,
-    int64_t nblocks, int split_stride, int vec_x, int vec_y, int rev)
+    const float *__restrict__ b, int K, int msteps, int m_hi, int64_t row0, int segs_per_row,
+    int64_t nblocks, int split_stride, int tab_offset, int copy_bytes, int vec_x, int vec_y,
+    int rev)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     constexpr int SEG_BLOCKS = kWaves * TW * 32;
@@ -245,6 +228,23 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
             }
         }
     }
+    // ---- tap tables: T_s[j] = split_s(b[C - j]), j in [0, L), two copies ----
+    // (C = 16*m_hi + 79 keeps every look-ahead read at j >= 0; L = C + 32.)
+    const int C = 16 * m_hi + 79, L = C + 32;
+    for (int q = threadIdx.x; q < 2 * (L + 1); q += kThreads) {
+        const int c = q / (L + 1), pos = q - c * (L + 1), j = pos - c;
+        const int k = C - j;
+        const float v = (j >= 0 && j < L && k >= 0 && k < K) ? b[k] : 0.0f;
+        unsigned short t[3];
+        if (S == 3)
+            split3(v, t[0], t[1], t[2]);
+        else
+            t[0] = f16_bits((_Float16)v);
+#pragma unroll
+        for (int s2 = 0; s2 < S; ++s2)
+            *reinterpret_cast<unsigned short *>(lds + tab_offset + (2 * s2 + c) * copy_bytes +
+                                                2 * pos) = t[s2];
+    }
     __syncthreads();
 
     // ---- contraction ----
@@ -264,23 +264,30 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
 #pragma unroll
     for (int t = 0; t < TW; ++t) ubase[t] = 4 * ((wave * TW + t) * 32 + p) + g + 2 * m_hi + 2;
 
-    // Software pipeline over half-steps, 4 per trip with no branches: tap
-    // fragments (L2-resident global memory) are fetched two half-steps ahead
-    // and LDS operands one half-step ahead, alternating between two register
-    // sets so no copies are needed.  The fragment buffer carries kLookahead
-    // zero steps past the last one and look-ahead LDS units are clamped to 0,
-    // so every load stays in bounds; the few remaining half-steps run after
-    // the loop.
-    const u32x4 *__restrict__ fr = frags + lane;
+    // Software pipeline over half-steps, 4 per trip with no branches: A
+    // operands are read two half-steps ahead and B operands one half-step
+    // ahead, alternating between two register sets so no copies are needed.
+    // The tap tables extend far enough for the look-ahead A reads (zeros) and
+    // look-ahead B units are clamped to 0, so every read stays in bounds; the
+    // few remaining half-steps run after the loop.
+    // Lane's A read for half-step mi starts at entry j0 = C + 16 - p + 8g - 16 mi
+    // of the copy whose shift c makes j0 + c even (4-byte aligned).
+    int abase[S];
+    {
+        const int c = (C - p) & 1;
+#pragma unroll
+        for (int s2 = 0; s2 < S; ++s2)
+            abase[s2] = tab_offset + (2 * s2 + c) * copy_bytes + 2 * (C + 16 - p + 8 * g + c);
+    }
     u32x4 aA[2][S], aB[2][S], bA[TW][S], bB[TW][S];
-    load_a<S>(aA, fr, 0);
+    load_a<S>(aA, lds, abase, 0);
     load_b<S, TW>(bA, lds, split_stride, ubase, 0);
     int mi = 0;
     // sched_barrier(0) fences keep each look-ahead load group ahead of the
     // MFMAs it must overlap (the scheduler would otherwise sink the loads next
     // to their first use and wait on them).
     for (; mi + 4 <= msteps; mi += 4) {
-        load_a<S>(aB, fr, mi + 2);
+        load_a<S>(aB, lds, abase, mi + 2);
         load_b<S, TW>(bB, lds, split_stride, ubase, mi + 1);
         __builtin_amdgcn_sched_barrier(0);
         step_mfma<S, TW>(acc0, acc1, aA[0], bA);
@@ -289,7 +296,7 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
         __builtin_amdgcn_sched_barrier(0);
         step_mfma<S, TW>(acc0, acc1, aA[1], bB);
         __builtin_amdgcn_sched_barrier(0);
-        load_a<S>(aA, fr, mi + 4);
+        load_a<S>(aA, lds, abase, mi + 4);
         load_b<S, TW>(bB, lds, split_stride, ubase, mi + 3);
         __builtin_amdgcn_sched_barrier(0);
         step_mfma<S, TW>(acc0, acc1, aB[0], bA);
@@ -304,7 +311,7 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
         load_b<S, TW>(bB, lds, split_stride, ubase, mi + 1);
         step_mfma<S, TW>(acc0, acc1, aA[0], bA);
         if (mi + 1 < msteps) {
-            load_a<S>(aB, fr, mi + 2);
+            load_a<S>(aB, lds, abase, mi + 2);
             load_b<S, TW>(bA, lds, split_stride, ubase, mi + 2);
             step_mfma<S, TW>(acc0, acc1, aA[1], bB);
             if (mi + 2 < msteps) step_mfma<S, TW>(acc0, acc1, aB[0], bA);
@@ -343,15 +350,27 @@ int align_ok(const float *p, int64_t ld)
     return ((reinterpret_cast<uintptr_t>(p) & 15) == 0) && (ld % 4 == 0);
 }
 
+// Bytes of one copy of one split's tap table (L + 1 entries), rounded so
+// consecutive copies start 128 bytes apart modulo the 256-byte LDS bank cycle.
+inline int tab_copy_bytes(int m_hi)
+{
+    const int L = 16 * m_hi + 79 + 32;
+    int bytes = (2 * (L + 1) + 15) & ~15;
+    while (bytes % 256 != 128) bytes += 16;
+    return bytes;
+}
+
 template <int S, int TW>
-int launch_mfma(const float *x, int64_t ldx, const u32x4 *frags, int K, int64_t B, int64_t N,
+int launch_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
                 float *y, int64_t ldy, bool rev, hipStream_t stream)
 {
     constexpr int SEG_BLOCKS = kWaves * TW * 32;
     const int m_hi = mfma_m_hi(K), msteps = mfma_steps(K);
     const int units = SEG_BLOCKS * 4 + 2 * m_hi;
     const int split_stride = (int)unit_bytes(units);
-    const size_t lds = (size_t)S * split_stride;
+    const int tab_offset = S * split_stride;
+    const int copy_bytes = tab_copy_bytes(m_hi);
+    const size_t lds = (size_t)tab_offset + (size_t)2 * S * copy_bytes;
     if (lds > (size_t)kMaxLds)
         return fail(SCIR_HIP_ERR_UNSUPPORTED, "K=%d needs %zu bytes of LDS for the MFMA path", K,
                     lds);
@@ -372,41 +391,25 @@ int launch_mfma(const float *x, int64_t ldx, const u32x4 *frags, int K, int64_t 
         const int64_t rows = std::min(rows_per_launch, B - r0);
         const int64_t nb = rows * segs;
         hipLaunchKernelGGL((fir_mfma_kernel<S, TW>), dim3((unsigned)nb), dim3(kThreads), lds,
-                           stream, x, ldx, N, y, ldy, frags, msteps, m_hi, r0, (int)segs, nb,
-                           split_stride, vec_x, vec_y, (int)rev);
+                           stream, x, ldx, N, y, ldy, b, K, msteps, m_hi, r0, (int)segs, nb,
+                           split_stride, tab_offset, copy_bytes, vec_x, vec_y, (int)rev);
         SCIR_TRY(check_launch("fir_mfma_kernel"));
     }
     return SCIR_HIP_OK;
 }
 
 template <int S>
-int launch_frags(const float *b, int K, u32x4 *frags, hipStream_t stream)
-{
-    const int msteps = mfma_steps(K) + kLookahead;  // zero steps for the look-ahead loads
-    const int threads = msteps * 64;
-    hipLaunchKernelGGL((fir_mfma_frags_kernel<S>), dim3((threads + 255) / 256), dim3(256), 0,
-                       stream, b, K, msteps, frags);
-    return check_launch("fir_mfma_frags_kernel");
-}
-
-template <int S>
 int run_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N, float *y,
-             int64_t ldy, bool rev, hipStream_t stream, DeviceState *st)
+             int64_t ldy, bool rev, hipStream_t stream)
 {
-    const int msteps = mfma_steps(K);
-    StreamScratch &sc = scratch_for(st, stream);
-    SCIR_TRY(ensure_buffer(sc.tap_frags, (size_t)(msteps + kLookahead) * S * 64 * sizeof(u32x4),
-                           stream));
-    u32x4 *frags = static_cast<u32x4 *>(sc.tap_frags.ptr);
-    SCIR_TRY(launch_frags<S>(b, K, frags, stream));
     const TuneChoice tc = tune_lookup(K, N);
     int tw = S == 3 ? tc.mfma_tiles : tc.mfma_tiles_fp16;
     // Short rows: smaller segments keep the grid full.
     if (N <= 4096) tw = 1;
     switch (tw) {
-    case 1: return launch_mfma<S, 1>(x, ldx, frags, K, B, N, y, ldy, rev, stream);
-    case 4: return launch_mfma<S, 4>(x, ldx, frags, K, B, N, y, ldy, rev, stream);
-    default: return launch_mfma<S, 2>(x, ldx, frags, K, B, N, y, ldy, rev, stream);
+    case 1: return launch_mfma<S, 1>(x, ldx, b, K, B, N, y, ldy, rev, stream);
+    case 4: return launch_mfma<S, 4>(x, ldx, b, K, B, N, y, ldy, rev, stream);
+    default: return launch_mfma<S, 2>(x, ldx, b, K, B, N, y, ldy, rev, stream);
     }
 }
 
@@ -418,10 +421,11 @@ int fir_mfma_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, 
 {
     if (B == 0 || N == 0) return SCIR_HIP_OK;
     if (K == 0) return fill_zero_2d(y, ldy, B, N, stream);
+    (void)st;
     if (precision == SCIR_HIP_PREC_BF16X3)
-        return run_mfma<3>(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
+        return run_mfma<3>(x, ldx, b, K, B, N, y, ldy, rev, stream);
     if (precision == SCIR_HIP_PREC_FP16)
-        return run_mfma<1>(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
+        return run_mfma<1>(x, ldx, b, K, B, N, y, ldy, rev, stream);
     return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "precision %d is not an MFMA mode", precision);
 }
 

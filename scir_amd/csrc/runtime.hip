@@ -213,7 +213,6 @@ int scir_hip_shutdown(void)
             b.bytes = 0;
         };
         for (auto &kv : up->scratch) {
-            release(kv.second.tap_frags);
             release(kv.second.partials);
         }
         up->scratch.clear();
