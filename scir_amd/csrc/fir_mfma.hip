@@ -8,16 +8,26 @@
 //   B_m[c][p] = x[32p - 16m + c]          (16 x 32 block columns)
 //   y[32p + r] = sum_m (A_m B_m)[r][p]
 // Each (k, output) pair appears exactly once (m = ceil((k-r)/16), c = 16m-k+r).
-// A_m depends only on the taps.  Each workgroup keeps the split taps in LDS
-// as a compact reversed table, T[j] = b[C - j]: lane (i, g) of A_m holds the
-// 8 consecutive entries starting at j0 = C - 16m - i + 8g, i.e. one 16-byte
-// read per split and half-step.  The table is stored twice, the second copy
-// shifted by one entry, so every lane's read is 4-byte aligned whatever the
-// parity of j0; the two copies are placed 128 bytes apart modulo the 256-byte
-// bank cycle so the lanes reading either copy do not collide.
-// B_m is a shifted view of the same staged x: no im2col copy is made; each
-// sample is split once into LDS and read by the ~2*(K+31)/32 half-steps that
-// need it.
+//
+// Operand reuse.  A wave owns TW output tiles of 32 blocks each, interleaved:
+// tile t covers blocks P + t + TW*i (i = 0..31, the MFMA column).  Then tile
+// t's B operand at half-step m equals tile 0's at half-step m - 2t, so the
+// loop runs over q and every iteration reads ONE B fragment D(q) (tile 0 at
+// half-step q) and applies it to tile t with A_{q+2t}.  The A fragments live
+// in a ring of 2*TW register sets: each is read once and used by TW tiles.
+// Per iteration and split term: one A and one B read for TW MFMAs.
+//
+// A comes from the taps only.  Each workgroup keeps the split taps in LDS as a
+// compact reversed table, T[j] = b[C - j]: lane (i, g) of A_m holds the 8
+// consecutive entries starting at j0 = C - 16m - i + 8g (one 16-byte read).
+// The table is stored twice, the second copy shifted by one entry, so every
+// lane's read is 4-byte aligned; the copies sit 128 bytes apart modulo the
+// 256-byte bank cycle.  Entries outside the taps are zero.
+//
+// B comes from x: the workgroup's segment of 4*TW*32 blocks plus 16*m_hi
+// samples of history is split once into LDS.  LDS unit u (8 samples, 16 B)
+// of a split array sits at byte 16*(u + u/(4*TW)): lanes read units 4*TW
+// apart, and this padding spreads 16 such lanes over distinct bank groups.
 //
 // Precision modes (template S = number of split terms):
 //   S = 3 (BF16x3): v = hi + mid + lo, each bf16 (hi = rn(v), mid = rn(v-hi),
@@ -25,17 +35,12 @@
 //     products are kept: hh (own accumulator), hm, mh, mm, hl, lh (a second
 //     accumulator); the dropped ml, lm, ll are below 2^-24 relative.
 //   S = 1 (FP16): x and b rounded to fp16, one product.
-//
-// Work decomposition: a workgroup (4 waves) owns a row segment of
-// 4*TW*32 blocks; each wave owns TW 32x32 output tiles (TW*1024 outputs).
-// The segment's inputs plus 16*m_hi samples of history are split into LDS
-// once.  LDS layout per split: unit u = 8 consecutive samples (16 B) at byte
-// 16*(u + u/4); a wave's B read has a 4-unit lane stride, which this padding
-// makes bank-conflict free.
+// Both assume finite inputs: a zero tap of the Toeplitz block still
+// multiplies its sample, so an Inf makes NaN in the outputs of its block.
 //
 // MFMA 32x32x16 operand layout (lane l = 32*g + i, g = l/32, i = l%32):
 //   A: lane holds A[i][8g + e], e = 0..7;  B: lane holds B[8g + e][i];
-//   D: lane holds D[8*(q) + 4g + j][i] in acc[4q + j], q, j = 0..3.
+//   D: lane holds D[8q + 4g + j][i] in acc[4q + j], q, j = 0..3.
 #include "common.hpp"
 
 #include <algorithm>
@@ -57,13 +62,29 @@ constexpr int kStageBatch = 4;  // LDS units (8 samples each) loaded per lane be
 inline int mfma_m_hi(int K) { return (K + 14) / 16; }  // ceil((K-1)/16) for K >= 1
 inline int mfma_steps(int K) { return mfma_m_hi(K) + 2; }
 
-__device__ __forceinline__ int unit_addr(int u) { return (u + (u >> 2)) * 16; }
-inline size_t unit_bytes(int units) { return (size_t)(units + (units >> 2) + 1) * 16; }
+// Byte address of LDS unit u in a split array whose lanes read 4*TW apart.
+template <int TW>
+__device__ __forceinline__ int unit_addr(int u)
+{
+    return (u + u / (4 * TW)) * 16;
+}
+inline size_t unit_bytes(int units, int tw) { return (size_t)(units + units / (4 * tw) + 1) * 16; }
+
+// Tap table: entries T[j] for j in [0, L), with C chosen so that every A read
+// of the loop (half-step indices -2*TW+2 .. msteps+2*TW-1) lands inside it.
+inline int tab_C(int m_hi, int tw) { return 16 * (m_hi + 2 * tw + 1) + 32; }
+inline int tab_L(int m_hi, int tw) { return tab_C(m_hi, tw) + 32 * tw + 32; }
+inline int tab_copy_bytes(int m_hi, int tw)
+{
+    int bytes = (2 * (tab_L(m_hi, tw) + 1) + 15) & ~15;
+    while (bytes % 256 != 128) bytes += 16;
+    return bytes;
+}
 
 __device__ __forceinline__ unsigned short bf16_bits(__bf16 v) { return __builtin_bit_cast(unsigned short, v); }
 __device__ __forceinline__ unsigned short f16_bits(_Float16 v) { return __builtin_bit_cast(unsigned short, v); }
 
-// Split term s (0 = hi, 1 = mid, 2 = lo) of v, as bf16 bits.
+// hi, mid, lo bf16 terms of v (v == hi + mid + lo exactly for normal v).
 __device__ __forceinline__ void split3(float v, unsigned short &h, unsigned short &m,
                                        unsigned short &l)
 {
@@ -92,24 +113,20 @@ __device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c)
                                                   __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
 }
 
-// One half-step for TW tiles: acc0 += hi(b)*hi(x); acc1 += the five
-// correction products (BF16x3), or acc0 += b*x (FP16).
-template <int S, int TW>
-__device__ __forceinline__ void step_mfma(f32x16 (&acc0)[TW], f32x16 (&acc1)[TW],
-                                          const u32x4 (&a)[S], const u32x4 (&bv)[TW][S])
+// acc0 += hi(b)*hi(x); acc1 += the five correction products (BF16x3), or
+// acc0 += b*x (FP16).
+template <int S>
+__device__ __forceinline__ void tile_mfma(f32x16 &acc0, f32x16 &acc1, const u32x4 (&a)[S],
+                                          const u32x4 (&d)[S])
 {
-#pragma unroll
-    for (int t = 0; t < TW; ++t) acc0[t] = mfma<S>(a[0], bv[t][0], acc0[t]);
+    acc0 = mfma<S>(a[0], d[0], acc0);
     if (S == 3) {
         constexpr int M = S > 1 ? 1 : 0, L = S > 2 ? 2 : 0;
-#pragma unroll
-        for (int t = 0; t < TW; ++t) {
-            acc1[t] = mfma<S>(a[0], bv[t][M], acc1[t]);  // hi(b) * mid(x)
-            acc1[t] = mfma<S>(a[M], bv[t][0], acc1[t]);  // mid(b) * hi(x)
-            acc1[t] = mfma<S>(a[M], bv[t][M], acc1[t]);  // mid(b) * mid(x)
-            acc1[t] = mfma<S>(a[0], bv[t][L], acc1[t]);  // hi(b) * lo(x)
-            acc1[t] = mfma<S>(a[L], bv[t][0], acc1[t]);  // lo(b) * hi(x)
-        }
+        acc1 = mfma<S>(a[0], d[M], acc1);  // hi(b) * mid(x)
+        acc1 = mfma<S>(a[M], d[0], acc1);  // mid(b) * hi(x)
+        acc1 = mfma<S>(a[M], d[M], acc1);  // mid(b) * mid(x)
+        acc1 = mfma<S>(a[0], d[L], acc1);  // hi(b) * lo(x)
+        acc1 = mfma<S>(a[L], d[0], acc1);  // lo(b) * hi(x)
     }
 }
 
@@ -118,34 +135,6 @@ __device__ __forceinline__ u32x4 lds_read16_a4(const unsigned char *p)
 {
     const unsigned int *q = reinterpret_cast<const unsigned int *>(p);
     return u32x4{q[0], q[1], q[2], q[3]};
-}
-
-// A operands of half-steps mi and mi+1: `abase[s]` is this lane's byte
-// address of half-step 0 in split s's table; each half-step moves 16 entries
-// (32 bytes) down the reversed table.
-template <int S>
-__device__ __forceinline__ void load_a(u32x4 (&a)[2][S], const unsigned char *lds,
-                                       const int (&abase)[S], int mi)
-{
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int s = 0; s < S; ++s) a[h][s] = lds_read16_a4(lds + abase[s] - 32 * (mi + h));
-}
-
-// B operands of half-step mi for TW tiles (unit clamped to 0 for look-ahead
-// steps past the last one, whose values are never used).
-template <int S, int TW>
-__device__ __forceinline__ void load_b(u32x4 (&bv)[TW][S], const unsigned char *lds,
-                                       int split_stride, const int (&ubase)[TW], int mi)
-{
-#pragma unroll
-    for (int t = 0; t < TW; ++t) {
-        const int u = max(ubase[t] - 2 * mi, 0);
-#pragma unroll
-        for (int s = 0; s < S; ++s)
-            bv[t][s] = *reinterpret_cast<const u32x4 *>(lds + s * split_stride + unit_addr(u));
-    }
 }
 
 // Workgroup id -> logical segment id so that the workgroups one XCD receives
@@ -157,6 +146,110 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb)
     const int64_t q = nb >> 3, rem = nb & 7;
     return xcd * q + (xcd < rem ? xcd : rem) + within;
 }
+
+// Loop state of one wave: accumulators, A ring, double-buffered B fragment.
+template <int S, int TW>
+struct Contraction {
+    static constexpr int RS = 2 * TW;  // A ring slots
+    f32x16 acc0[TW], acc1[TW];
+    u32x4 ring[RS][S];
+    u32x4 dA[S], dB[S];
+    const unsigned char *lds;
+    int split_stride;
+    int ubase;      // B unit of iteration 0
+    int abase[S];   // A table byte address of half-step index 0
+    int msteps, Q;  // half-steps per tile, loop iterations
+
+    __device__ __forceinline__ void load_a(u32x4 (&a)[S], int sa) const
+    {
+#pragma unroll
+        for (int s = 0; s < S; ++s) a[s] = lds_read16_a4(lds + abase[s] - 32 * sa);
+    }
+    __device__ __forceinline__ void load_d(u32x4 (&d)[S], int n) const
+    {
+        const int u = max(ubase - 2 * n, 0);  // past the last iteration: any in-range unit
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            d[s] = *reinterpret_cast<const u32x4 *>(lds + s * split_stride + unit_addr<TW>(u));
+    }
+
+    // Iteration n (n % RS == KK): tile t uses A of half-step index
+    // sa = n - 2*TW + 2 + 2t (ring slot (KK + 2 + 2t) % RS) and D(n).  After
+    // tile 0, whose A is then dead, the slot is refilled with half-step n + 2.
+    // With PRED, a tile whose sa is outside [0, msteps) is skipped.
+    template <int KK, bool PRED>
+    __device__ __forceinline__ void step(int n)
+    {
+        u32x4(&dcur)[S] = (KK & 1) ? dB : dA;
+        u32x4(&dnxt)[S] = (KK & 1) ? dA : dB;
+        load_d(dnxt, n + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        {
+            constexpr int slot = (KK + 2) % RS;
+            const int sa = n - 2 * TW + 2;
+            if (!PRED || (sa >= 0 && sa < msteps)) tile_mfma<S>(acc0[0], acc1[0], ring[slot], dcur);
+            __builtin_amdgcn_sched_barrier(0);
+            load_a(ring[slot], n + 2);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int t = 1; t < TW; ++t) {
+            const int slot = (KK + 2 + 2 * t) % RS;
+            const int sa = n - 2 * TW + 2 + 2 * t;
+            if (!PRED || (sa >= 0 && sa < msteps)) tile_mfma<S>(acc0[t], acc1[t], ring[slot], dcur);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+
+    template <int KK>
+    __device__ __forceinline__ void tail(int n)
+    {
+        if constexpr (KK < RS) {
+            if (n < Q) {
+                step<KK, true>(n);
+                tail<KK + 1>(n + 1);
+            }
+        }
+    }
+
+    template <int KK, bool PRED>
+    __device__ __forceinline__ void body(int n0)
+    {
+        if constexpr (KK < RS) {
+            step<KK, PRED>(n0 + KK);
+            body<KK + 1, PRED>(n0);
+        }
+    }
+
+    __device__ __forceinline__ void run()
+    {
+#pragma unroll
+        for (int t = 0; t < TW; ++t)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc0[t][i] = acc1[t][i] = 0.0f;
+        // The ring at iteration 0 holds half-step indices -2TW+2 .. 1 in slot
+        // (index mod RS); negative indices read zero table entries.
+#pragma unroll
+        for (int k = 0; k < RS; ++k) {
+            const int sa = -2 * TW + 2 + k;
+            load_a(ring[(k + 2) % RS], sa);  // (sa mod RS) == (k + 2) mod RS
+        }
+        load_d(dA, 0);
+        // The first and last 2TW-2 iterations have tiles outside their
+        // half-steps: they run predicated; the rest as unrolled RS-iteration
+        // bodies with no branches.
+        const int ramp = 2 * TW - 2;
+        int n0 = 0;
+        if (ramp > 0 && Q >= RS) {
+            body<0, true>(0);
+            n0 = RS;
+        }
+        for (; n0 + RS <= Q - ramp; n0 += RS) body<0, false>(n0);
+        // At most RS - 1 + ramp < 2*RS iterations remain.
+        tail<0>(n0);
+        tail<0>(n0 + RS);
+    }
+};
 
 template <int S, int TW>
 __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
@@ -224,13 +317,13 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
             for (int s = 0; s < S; ++s) {
                 u32x4 w = {pack2(t[s][0], t[s][1]), pack2(t[s][2], t[s][3]),
                            pack2(t[s][4], t[s][5]), pack2(t[s][6], t[s][7])};
-                *reinterpret_cast<u32x4 *>(lds + s * split_stride + unit_addr(u)) = w;
+                *reinterpret_cast<u32x4 *>(lds + s * split_stride + unit_addr<TW>(u)) = w;
             }
         }
     }
+
     // ---- tap tables: T_s[j] = split_s(b[C - j]), j in [0, L), two copies ----
-    // (C = 16*m_hi + 79 keeps every look-ahead read at j >= 0; L = C + 32.)
-    const int C = 16 * m_hi + 79, L = C + 32;
+    const int C = 16 * (m_hi + 2 * TW + 1) + 32, L = C + 32 * TW + 32;
     for (int q = threadIdx.x; q < 2 * (L + 1); q += kThreads) {
         const int c = q / (L + 1), pos = q - c * (L + 1), j = pos - c;
         const int k = C - j;
@@ -249,87 +342,37 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
 
     // ---- contraction ----
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int p = lane & 31, g = lane >> 5;
-    f32x16 acc0[TW], acc1[TW];
-#pragma unroll
-    for (int t = 0; t < TW; ++t) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            acc0[t][i] = 0.0f;
-            acc1[t][i] = 0.0f;
-        }
-    }
-    // unit of B_m for tile t: 4*(tile block offset + p) + 2*(m_hi - m) + g
-    int ubase[TW];
-#pragma unroll
-    for (int t = 0; t < TW; ++t) ubase[t] = 4 * ((wave * TW + t) * 32 + p) + g + 2 * m_hi + 2;
-
-    // Software pipeline over half-steps, 4 per trip with no branches: A
-    // operands are read two half-steps ahead and B operands one half-step
-    // ahead, alternating between two register sets so no copies are needed.
-    // The tap tables extend far enough for the look-ahead A reads (zeros) and
-    // look-ahead B units are clamped to 0, so every read stays in bounds; the
-    // few remaining half-steps run after the loop.
-    // Lane's A read for half-step mi starts at entry j0 = C + 16 - p + 8g - 16 mi
-    // of the copy whose shift c makes j0 + c even (4-byte aligned).
-    int abase[S];
+    const int i = lane & 31, g = lane >> 5;
+    Contraction<S, TW> ct;
+    ct.lds = lds;
+    ct.split_stride = split_stride;
+    ct.msteps = msteps;
+    ct.Q = msteps + 2 * TW - 2;
+    // D(n) = tile 0's B at half-step q = n - 2TW + 1: unit
+    // 4*(wave offset + TW*i) + 2*(m_hi - q) + g.
+    ct.ubase = 4 * (wave * 32 * TW + TW * i) + 2 * (m_hi + 2 * TW - 1) + g;
     {
-        const int c = (C - p) & 1;
+        // A of half-step index sa (m = sa - 1) starts at entry
+        // j0 = C - 16*(sa - 1) - i + 8g, in the copy whose shift c makes j0 + c even.
+        const int c = (C - i) & 1;
 #pragma unroll
         for (int s2 = 0; s2 < S; ++s2)
-            abase[s2] = tab_offset + (2 * s2 + c) * copy_bytes + 2 * (C + 16 - p + 8 * g + c);
+            ct.abase[s2] = tab_offset + (2 * s2 + c) * copy_bytes + 2 * (C + 16 - i + 8 * g + c);
     }
-    u32x4 aA[2][S], aB[2][S], bA[TW][S], bB[TW][S];
-    load_a<S>(aA, lds, abase, 0);
-    load_b<S, TW>(bA, lds, split_stride, ubase, 0);
-    int mi = 0;
-    // sched_barrier(0) fences keep each look-ahead load group ahead of the
-    // MFMAs it must overlap (the scheduler would otherwise sink the loads next
-    // to their first use and wait on them).
-    for (; mi + 4 <= msteps; mi += 4) {
-        load_a<S>(aB, lds, abase, mi + 2);
-        load_b<S, TW>(bB, lds, split_stride, ubase, mi + 1);
-        __builtin_amdgcn_sched_barrier(0);
-        step_mfma<S, TW>(acc0, acc1, aA[0], bA);
-        __builtin_amdgcn_sched_barrier(0);
-        load_b<S, TW>(bA, lds, split_stride, ubase, mi + 2);
-        __builtin_amdgcn_sched_barrier(0);
-        step_mfma<S, TW>(acc0, acc1, aA[1], bB);
-        __builtin_amdgcn_sched_barrier(0);
-        load_a<S>(aA, lds, abase, mi + 4);
-        load_b<S, TW>(bB, lds, split_stride, ubase, mi + 3);
-        __builtin_amdgcn_sched_barrier(0);
-        step_mfma<S, TW>(acc0, acc1, aB[0], bA);
-        __builtin_amdgcn_sched_barrier(0);
-        load_b<S, TW>(bA, lds, split_stride, ubase, mi + 4);
-        __builtin_amdgcn_sched_barrier(0);
-        step_mfma<S, TW>(acc0, acc1, aB[1], bB);
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    // aA holds half-steps mi, mi+1 and bA half-step mi.
-    if (mi < msteps) {
-        load_b<S, TW>(bB, lds, split_stride, ubase, mi + 1);
-        step_mfma<S, TW>(acc0, acc1, aA[0], bA);
-        if (mi + 1 < msteps) {
-            load_a<S>(aB, lds, abase, mi + 2);
-            load_b<S, TW>(bA, lds, split_stride, ubase, mi + 2);
-            step_mfma<S, TW>(acc0, acc1, aA[1], bB);
-            if (mi + 2 < msteps) step_mfma<S, TW>(acc0, acc1, aB[0], bA);
-        }
-    }
+    ct.run();
 
-    // ---- epilogue: y[32*(block) + 8q + 4g + j] = acc[4q + j] ----
+    // ---- epilogue: tile t, lane (i, g): block P + t + TW*i, sample 8q + 4g + j ----
     float *__restrict__ yrow = y + row * ldy;
 #pragma unroll
     for (int t = 0; t < TW; ++t) {
-        const int64_t n0 = 32 * (P0 + (wave * TW + t) * 32 + p) + 4 * g;
+        const int64_t n0 = 32 * (P0 + wave * 32 * TW + t + TW * i) + 4 * g;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             f32x4 v;
-            v.x = acc0[t][4 * q + 0] + acc1[t][4 * q + 0];
-            v.y = acc0[t][4 * q + 1] + acc1[t][4 * q + 1];
-            v.z = acc0[t][4 * q + 2] + acc1[t][4 * q + 2];
-            v.w = acc0[t][4 * q + 3] + acc1[t][4 * q + 3];
+            v.x = ct.acc0[t][4 * q + 0] + ct.acc1[t][4 * q + 0];
+            v.y = ct.acc0[t][4 * q + 1] + ct.acc1[t][4 * q + 1];
+            v.z = ct.acc0[t][4 * q + 2] + ct.acc1[t][4 * q + 2];
+            v.w = ct.acc0[t][4 * q + 3] + ct.acc1[t][4 * q + 3];
             const int64_t n = n0 + 8 * q;
             if (vec_y && n + 3 < N) {
                 if (!rev)
@@ -350,14 +393,12 @@ int align_ok(const float *p, int64_t ld)
     return ((reinterpret_cast<uintptr_t>(p) & 15) == 0) && (ld % 4 == 0);
 }
 
-// Bytes of one copy of one split's tap table (L + 1 entries), rounded so
-// consecutive copies start 128 bytes apart modulo the 256-byte LDS bank cycle.
-inline int tab_copy_bytes(int m_hi)
+template <int S, int TW>
+size_t mfma_lds_bytes(int K)
 {
-    const int L = 16 * m_hi + 79 + 32;
-    int bytes = (2 * (L + 1) + 15) & ~15;
-    while (bytes % 256 != 128) bytes += 16;
-    return bytes;
+    const int m_hi = mfma_m_hi(K);
+    const int units = kWaves * TW * 32 * 4 + 2 * m_hi;
+    return (size_t)S * unit_bytes(units, TW) + (size_t)2 * S * tab_copy_bytes(m_hi, TW);
 }
 
 template <int S, int TW>
@@ -367,13 +408,10 @@ int launch_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, i
     constexpr int SEG_BLOCKS = kWaves * TW * 32;
     const int m_hi = mfma_m_hi(K), msteps = mfma_steps(K);
     const int units = SEG_BLOCKS * 4 + 2 * m_hi;
-    const int split_stride = (int)unit_bytes(units);
+    const int split_stride = (int)unit_bytes(units, TW);
     const int tab_offset = S * split_stride;
-    const int copy_bytes = tab_copy_bytes(m_hi);
-    const size_t lds = (size_t)tab_offset + (size_t)2 * S * copy_bytes;
-    if (lds > (size_t)kMaxLds)
-        return fail(SCIR_HIP_ERR_UNSUPPORTED, "K=%d needs %zu bytes of LDS for the MFMA path", K,
-                    lds);
+    const int copy_bytes = tab_copy_bytes(m_hi, TW);
+    const size_t lds = mfma_lds_bytes<S, TW>(K);
     static bool attr_set = false;  // guarded by the device mutex held by the caller
     if (!attr_set) {
         SCIR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&fir_mfma_kernel<S, TW>),
@@ -406,6 +444,11 @@ int run_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int6
     int tw = S == 3 ? tc.mfma_tiles : tc.mfma_tiles_fp16;
     // Short rows: smaller segments keep the grid full.
     if (N <= 4096) tw = 1;
+    // Long filters: fewer tiles per wave when the LDS would overflow.
+    if (tw >= 4 && mfma_lds_bytes<S, 4>(K) > (size_t)kMaxLds) tw = 2;
+    if (tw >= 2 && mfma_lds_bytes<S, 2>(K) > (size_t)kMaxLds) tw = 1;
+    if (mfma_lds_bytes<S, 1>(K) > (size_t)kMaxLds)
+        return fail(SCIR_HIP_ERR_UNSUPPORTED, "K=%d needs more LDS than the MFMA path has", K);
     switch (tw) {
     case 1: return launch_mfma<S, 1>(x, ldx, b, K, B, N, y, ldy, rev, stream);
     case 4: return launch_mfma<S, 4>(x, ldx, b, K, B, N, y, ldy, rev, stream);
@@ -417,11 +460,10 @@ int run_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int6
 
 int fir_mfma_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
                  float *y, int64_t ldy, int precision, bool rev, hipStream_t stream,
-                 DeviceState *st)
+                 DeviceState *)
 {
     if (B == 0 || N == 0) return SCIR_HIP_OK;
     if (K == 0) return fill_zero_2d(y, ldy, B, N, stream);
-    (void)st;
     if (precision == SCIR_HIP_PREC_BF16X3)
         return run_mfma<3>(x, ldx, b, K, B, N, y, ldy, rev, stream);
     if (precision == SCIR_HIP_PREC_FP16)
