@@ -57,7 +57,7 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
 constexpr int kMaxLds = 160 * 1024;
-constexpr int kStageBatch = 4;  // LDS units (8 samples each) loaded per lane before splitting
+constexpr int kStageBatch = 8;  // LDS units (8 samples each) loaded per lane before splitting
 
 inline int mfma_m_hi(int K) { return (K + 14) / 16; }  // ceil((K-1)/16) for K >= 1
 inline int mfma_steps(int K) { return mfma_m_hi(K) + 2; }
@@ -267,9 +267,29 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
     const int64_t xb = 32 * P0 - 16 * (int64_t)m_hi;        // sample index of LDS unit 0
     const int units = SEG_BLOCKS * 4 + 2 * m_hi;
 
+    // ---- tap tables: T_s[j] = split_s(b[C - j]), j in [0, L), two copies ----
+    const int C = 16 * (m_hi + 2 * TW + 1) + 32, L = C + 32 * TW + 32;
+    auto build_tables = [&]() {
+        for (int q = threadIdx.x; q < 2 * (L + 1); q += kThreads) {
+            const int c = q / (L + 1), pos = q - c * (L + 1), j = pos - c;
+            const int k = C - j;
+            const float v = (j >= 0 && j < L && k >= 0 && k < K) ? b[k] : 0.0f;
+            unsigned short t[3];
+            if (S == 3)
+                split3(v, t[0], t[1], t[2]);
+            else
+                t[0] = f16_bits((_Float16)v);
+#pragma unroll
+            for (int s2 = 0; s2 < S; ++s2)
+                *reinterpret_cast<unsigned short *>(lds + tab_offset + (2 * s2 + c) * copy_bytes +
+                                                    2 * pos) = t[s2];
+        }
+    };
+
     // ---- stage: x[xb .. xb + 8*units) -> S split arrays in LDS ----
     // Each lane issues the loads of kStageBatch units before splitting any of
-    // them, so their memory latencies overlap.
+    // them, so their memory latencies overlap; the tap tables are built while
+    // the first batch is in flight.
     for (int u0 = threadIdx.x; u0 < units; u0 += kStageBatch * kThreads) {
         f32x4 raw[kStageBatch][2];
 #pragma unroll
@@ -300,6 +320,7 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
                 }
             }
         }
+        if (u0 == (int)threadIdx.x) build_tables();
 #pragma unroll
         for (int j = 0; j < kStageBatch; ++j) {
             const int u = u0 + j * kThreads;
@@ -322,22 +343,6 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
         }
     }
 
-    // ---- tap tables: T_s[j] = split_s(b[C - j]), j in [0, L), two copies ----
-    const int C = 16 * (m_hi + 2 * TW + 1) + 32, L = C + 32 * TW + 32;
-    for (int q = threadIdx.x; q < 2 * (L + 1); q += kThreads) {
-        const int c = q / (L + 1), pos = q - c * (L + 1), j = pos - c;
-        const int k = C - j;
-        const float v = (j >= 0 && j < L && k >= 0 && k < K) ? b[k] : 0.0f;
-        unsigned short t[3];
-        if (S == 3)
-            split3(v, t[0], t[1], t[2]);
-        else
-            t[0] = f16_bits((_Float16)v);
-#pragma unroll
-        for (int s2 = 0; s2 < S; ++s2)
-            *reinterpret_cast<unsigned short *>(lds + tab_offset + (2 * s2 + c) * copy_bytes +
-                                                2 * pos) = t[s2];
-    }
     __syncthreads();
 
     // ---- contraction ----
