@@ -49,6 +49,9 @@ const int64_t kNBuckets[][2] = {{0, 8192}, {8193, (int64_t)1 << 62}};
 TuneChoice tune_lookup(int K, int64_t N)
 {
     TuneChoice c;
+    // Built-in defaults (measured on MI355X): long filters share each tap
+    // fragment across four tiles per wave.
+    if (K > 1017) c.mfma_tiles = 4;
     std::lock_guard<std::mutex> g(g_tune_mu);
     for (const Entry &e : g_table)
         if (K >= e.k_lo && K <= e.k_hi && N >= e.n_lo && N <= e.n_hi) {

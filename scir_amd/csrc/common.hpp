@@ -103,7 +103,7 @@ struct TuneChoice {
     int direct_seg_tiles = 2; // tiles per workgroup of the pipelined variant
     int mfma_tiles = 2;       // 32x32 output tiles per wave, BF16x3 MFMA kernel
     int mfma_tiles_fp16 = 4;  // the same for the FP16 MFMA kernel
-    int auto_mfma_min_k = 160; // AUTO uses BF16x3 MFMA from this K up
+    int auto_mfma_min_k = 96; // AUTO uses BF16x3 MFMA from this K up
 };
 TuneChoice tune_lookup(int K, int64_t N);
 
