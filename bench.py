@@ -259,6 +259,9 @@ def run_ragged(args):
     out["roofline"] = dict(bound="fp32", achieved=round(ach, 2), peak=FP32_PEAK_TFLOPS,
                            unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 4),
                            traffic=load_traffic("fir_ragged_f32_ragged_pmc"))
+    if args.quick:
+        plan.close()
+        return out
     ms_s, _, _ = timed(lambda: plan.execute_per_signal(x, bt, y), 1, 1, stream)
     side = torch.cuda.Stream()
     torch.cuda.synchronize()
@@ -332,11 +335,12 @@ def main():
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default=HEADLINE,
                     choices=sorted(DENSE) + ["ragged", "vjp_k255", "copy_1gib"])
     ap.add_argument("--precision", default=None)
-    ap.add_argument("--quick", action="store_true", help="skip the side comparisons")
+    ap.add_argument("--quick", action="store_true",
+                    help="skip the side comparisons (and, for ragged, the per-signal launches)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     args = ap.parse_args()
