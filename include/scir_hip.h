@@ -179,6 +179,8 @@ SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
  *   "direct.loader"   0 = one tile per workgroup, 1 = pipelined row segments
  *   "direct.seg_tiles" tiles per workgroup of the pipelined direct kernel
  *   "mfma.tiles"      32x32 output tiles per wave of the MFMA kernel (1, 2, 4)
+ *   "mfma.segs"       row segments per workgroup of the MFMA kernel (>= 1); the
+ *                     next segment's samples load while the current one computes
  *   "auto.mfma_min_k" smallest K for which AUTO picks BF16x3 MFMA
  * The autotuner's search uses the same mechanism. */
 SCIR_HIP_API int scir_hip_tune_set(const char *key, int32_t value);

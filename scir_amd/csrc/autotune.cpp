@@ -3,15 +3,16 @@
 // The table maps (K bucket, N bucket) to launch shapes:
 //   direct_r / direct_loader / direct_seg_tiles   FP32 direct kernel
 //   mfma_tiles / mfma_tiles_fp16                  MFMA kernel tiles per wave
+//   mfma_segs                                     MFMA row segments per workgroup
 // plus one global AUTO cut-over K (F32 direct below, BF16x3 MFMA from there).
 //
 // scir_hip_autotune() fills it by timing every candidate with HIP events on
-// synthetic rows (2^24 samples per bucket, N = the bucket's representative
+// synthetic rows (2^26 samples per bucket, N = the bucket's representative
 // row length) on the current device, keeps the fastest, and writes
 //   {"arch": "gfx950", "auto_mfma_min_k": 160,
 //    "entries": [{"k_lo": 1, "k_hi": 16, "n_lo": 0, "n_hi": 8192,
 //                 "direct_r": 8, "direct_loader": 1, "direct_seg_tiles": 2,
-//                 "mfma_tiles": 2, "mfma_tiles_fp16": 4}, ...]}
+//                 "mfma_tiles": 2, "mfma_tiles_fp16": 4, "mfma_segs": 16}, ...]}
 // scir_hip_autotune_load() reads that format back (a table written for
 // another arch is ignored).  Overrides set with scir_hip_tune_set() win over
 // the table.
@@ -38,7 +39,7 @@ std::vector<Entry> g_table;
 int g_auto_min_k = -1;  // from the table; -1 = built-in default
 // Overrides set through scir_hip_tune_set (-1 = unset).
 int g_over_direct_r = -1, g_over_direct_loader = -1, g_over_direct_seg = -1,
-    g_over_mfma_tiles = -1, g_over_auto_min_k = -1;
+    g_over_mfma_tiles = -1, g_over_auto_min_k = -1, g_over_mfma_segs = -1;
 
 // Buckets searched by the autotuner.
 const int kKBuckets[][2] = {{1, 16}, {17, 64}, {65, 256}, {257, 1017}, {1018, 4096}};
@@ -64,6 +65,10 @@ TuneChoice tune_lookup(int K, int64_t N)
     if (g_over_direct_seg >= 0) c.direct_seg_tiles = g_over_direct_seg;
     if (g_over_mfma_tiles >= 0) c.mfma_tiles = c.mfma_tiles_fp16 = g_over_mfma_tiles;
     if (g_over_auto_min_k >= 0) c.auto_mfma_min_k = g_over_auto_min_k;
+    if (g_over_mfma_segs >= 0) {
+        c.mfma_segs = g_over_mfma_segs;
+        c.mfma_segs_forced = true;
+    }
     return c;
 }
 
@@ -115,7 +120,8 @@ std::string table_json(const std::string &arch)
           << ", \"direct_r\": " << e.c.direct_r << ", \"direct_loader\": " << e.c.direct_loader
           << ", \"direct_seg_tiles\": " << e.c.direct_seg_tiles
           << ", \"mfma_tiles\": " << e.c.mfma_tiles
-          << ", \"mfma_tiles_fp16\": " << e.c.mfma_tiles_fp16 << "}";
+          << ", \"mfma_tiles_fp16\": " << e.c.mfma_tiles_fp16
+          << ", \"mfma_segs\": " << e.c.mfma_segs << "}";
     }
     o << "]}\n";
     return o.str();
@@ -175,6 +181,9 @@ extern "C" int scir_hip_tune_set(const char *key, int32_t value)
         g_over_mfma_tiles = v;
     } else if (!std::strcmp(key, "auto.mfma_min_k")) {
         g_over_auto_min_k = v;
+    } else if (!std::strcmp(key, "mfma.segs")) {
+        if (v == 0) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "mfma.segs must be >= 1");
+        g_over_mfma_segs = v;
     } else {
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "unknown tuning key '%s'", key);
     }
@@ -220,6 +229,8 @@ extern "C" int scir_hip_autotune_load(const char *cache_path)
         e.c.direct_seg_tiles = (int)std::max<int64_t>(1, v[6]);
         e.c.mfma_tiles = (v[7] == 1 || v[7] == 4) ? (int)v[7] : 2;
         e.c.mfma_tiles_fp16 = (v[8] == 1 || v[8] == 2) ? (int)v[8] : 4;
+        int64_t segs = 0;  // optional (tables written before the key existed)
+        if (find_int(s, a, b, "mfma_segs", &segs) && segs >= 1) e.c.mfma_segs = (int)segs;
         table.push_back(e);
         p = b;
     }
@@ -242,7 +253,7 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
                                                          t_start).count() > budget;
     };
-    const int64_t total = (int64_t)1 << 24;  // samples per timing problem
+    const int64_t total = (int64_t)1 << 26;  // samples per timing problem
     const int64_t long_n = (int64_t)1 << 18;
     hipStream_t s = nullptr;
     SCIR_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -269,7 +280,7 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
             status = fail(SCIR_HIP_ERR_BACKEND, "autotune tap upload failed");
     }
     // Search with the table cleared; overrides drive each candidate.
-    int saved[5];
+    int saved[6];
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         saved[0] = g_over_direct_r;
@@ -277,15 +288,17 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
         saved[2] = g_over_direct_seg;
         saved[3] = g_over_mfma_tiles;
         saved[4] = g_over_auto_min_k;
+        saved[5] = g_over_mfma_segs;
         g_table.clear();
         g_auto_min_k = -1;
     }
-    auto set = [&](int r, int loader, int seg, int tiles) {
+    auto set = [&](int r, int loader, int seg, int tiles, int msegs) {
         std::lock_guard<std::mutex> g(g_tune_mu);
         g_over_direct_r = r;
         g_over_direct_loader = loader;
         g_over_direct_seg = seg;
         g_over_mfma_tiles = tiles;
+        g_over_mfma_segs = msegs;
         g_over_auto_min_k = -1;
     };
     std::vector<Entry> found;
@@ -306,7 +319,7 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
                 for (int ld : loaders)
                     for (int sg : segs) {
                         if ((ld == 0 && sg != 1) || status != SCIR_HIP_OK) continue;
-                        set(r, ld, sg, -1);
+                        set(r, ld, sg, -1, -1);
                         float ms = 0;
                         status = time_ms([&]() {
                             return scir_hip_fir_f32_dev(x, N, b, K, B, N, y, N,
@@ -325,7 +338,7 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
                 float bt = 1e30f;
                 for (int tw : tiles) {
                     if (status != SCIR_HIP_OK) break;
-                    set(-1, -1, -1, tw);
+                    set(-1, -1, -1, tw, -1);
                     float ms = 0;
                     status = time_ms([&]() {
                         return scir_hip_fir_f32_dev(x, N, b, K, B, N, y, N, precs[pi], s);
@@ -333,6 +346,24 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
                     if (status == SCIR_HIP_OK && ms < bt) {
                         bt = ms;
                         (pi == 0 ? e.c.mfma_tiles : e.c.mfma_tiles_fp16) = tw;
+                    }
+                }
+            }
+            // Row segments per MFMA workgroup, with the BF16x3 tile choice.
+            {
+                float bt = 1e30f;
+                const int msegs[3] = {1, 4, 8};
+                for (int sg : msegs) {
+                    if (status != SCIR_HIP_OK) break;
+                    set(-1, -1, -1, e.c.mfma_tiles, sg);
+                    float ms = 0;
+                    status = time_ms([&]() {
+                        return scir_hip_fir_f32_dev(x, N, b, K, B, N, y, N,
+                                                    SCIR_HIP_PREC_BF16X3, s);
+                    }, s, 3, &ms);
+                    if (status == SCIR_HIP_OK && ms < bt) {
+                        bt = ms;
+                        e.c.mfma_segs = sg;
                     }
                 }
             }
@@ -346,7 +377,7 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
             std::lock_guard<std::mutex> g(g_tune_mu);
             g_table = found;
         }
-        set(-1, -1, -1, -1);
+        set(-1, -1, -1, -1, -1);
         const int ks[] = {48, 64, 96, 128, 160, 192, 256, 384, 512};
         cut = -1;
         for (int K : ks) {
@@ -374,6 +405,7 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
         g_over_direct_seg = saved[2];
         g_over_mfma_tiles = saved[3];
         g_over_auto_min_k = saved[4];
+        g_over_mfma_segs = saved[5];
         if (status == SCIR_HIP_OK) {
             g_table = found;
             g_auto_min_k = cut;

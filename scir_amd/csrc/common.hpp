@@ -100,9 +100,11 @@ int fill_zero_2d(float *y, int64_t ldy, int64_t B, int64_t N, hipStream_t stream
 struct TuneChoice {
     int direct_r = 8;         // outputs per lane in the direct kernel
     int direct_loader = 1;    // 0: one tile per workgroup, 1: pipelined row segments
-    int direct_seg_tiles = 2; // tiles per workgroup of the pipelined variant
+    int direct_seg_tiles = 8; // tiles per workgroup of the pipelined variant
     int mfma_tiles = 2;       // 32x32 output tiles per wave, BF16x3 MFMA kernel
     int mfma_tiles_fp16 = 4;  // the same for the FP16 MFMA kernel
+    int mfma_segs = 16;       // row segments per workgroup of the MFMA kernel
+    bool mfma_segs_forced = false;  // set through scir_hip_tune_set: no grid-size cap
     int auto_mfma_min_k = 96; // AUTO uses BF16x3 MFMA from this K up
 };
 TuneChoice tune_lookup(int K, int64_t N);

@@ -57,7 +57,9 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
 constexpr int kMaxLds = 160 * 1024;
-constexpr int kStageBatch = 8;  // LDS units (8 samples each) loaded per lane before splitting
+// LDS units (8 samples each) per lane loaded together when a segment's history
+// is longer than the prefetched part (2*m_hi > 256 units, K > 2049).
+constexpr int kRestBatch = 2;
 
 inline int mfma_m_hi(int K) { return (K + 14) / 16; }  // ceil((K-1)/16) for K >= 1
 inline int mfma_steps(int K) { return mfma_m_hi(K) + 2; }
@@ -251,25 +253,104 @@ struct Contraction {
     }
 };
 
+// Samples of LDS unit u (8 samples from xb + 8u of the row, logical order;
+// zero outside [0, N) and for u >= units).  In rev mode logical sample i is
+// physical sample N-1-i.
+__device__ __forceinline__ void load_unit(const float *__restrict__ xrow, int64_t N, int64_t xb,
+                                          int u, int units, int vec_x, int rev, f32x4 (&r)[2])
+{
+    const int64_t g0 = xb + 8 * (int64_t)u;
+    if (u >= units) {
+        r[0] = r[1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    } else if (vec_x && g0 >= 0 && g0 + 7 < N) {
+        if (!rev) {
+            r[0] = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0));
+            r[1] = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0 + 4));
+        } else {  // logical g0..g0+7 == physical N-1-g0 .. N-8-g0
+            const f32x4 a =
+                __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + (N - 4 - g0)));
+            const f32x4 c =
+                __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + (N - 8 - g0)));
+            r[0] = f32x4{a.w, a.z, a.y, a.x};
+            r[1] = f32x4{c.w, c.z, c.y, c.x};
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int64_t i = g0 + e;
+            r[e >> 2][e & 3] = (i >= 0 && i < N) ? xrow[rev ? N - 1 - i : i] : 0.0f;
+        }
+    }
+}
+
+// Split the 8 samples of unit u into the S split arrays of the LDS.
+template <int S, int TW>
+__device__ __forceinline__ void store_unit(unsigned char *lds, int split_stride, int u,
+                                           const f32x4 (&r)[2])
+{
+    unsigned short t[S][8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const float v = r[e >> 2][e & 3];
+        if (S == 3)
+            split3(v, t[0][e], t[S > 1 ? 1 : 0][e], t[S > 2 ? 2 : 0][e]);
+        else
+            t[0][e] = f16_bits((_Float16)v);
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        u32x4 w = {pack2(t[s][0], t[s][1]), pack2(t[s][2], t[s][3]), pack2(t[s][4], t[s][5]),
+                   pack2(t[s][6], t[s][7])};
+        *reinterpret_cast<u32x4 *>(lds + s * split_stride + unit_addr<TW>(u)) = w;
+    }
+}
+
+// One workgroup (4 waves) computes segs_per_wg consecutive segments of
+// kWaves*TW*32 output blocks each (segment id = row * segs_per_row + index in
+// the row).  A segment needs LDS units 0 .. units-1: 2*m_hi units of history
+// and 4*SEG_BLOCKS of its own samples.  The first PF = 2*TW + 1 units of
+// each lane (unit lane + 256*j) are loaded into registers one segment ahead:
+// they are in flight while the waves run the previous segment's MFMAs.
+// Units past PF*256 (only when 2*m_hi > 256) are loaded when the segment is
+// staged.
 template <int S, int TW>
 __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
     const float *__restrict__ x, int64_t ldx, int64_t N, float *__restrict__ y, int64_t ldy,
     const float *__restrict__ b, int K, int msteps, int m_hi, int64_t row0, int segs_per_row,
-    int64_t nblocks, int split_stride, int tab_offset, int copy_bytes, int vec_x, int vec_y,
-    int rev)
+    int64_t nsegs, int segs_per_wg, int64_t nwg, int split_stride, int tab_offset,
+    int copy_bytes, int vec_x, int vec_y, int rev)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     constexpr int SEG_BLOCKS = kWaves * TW * 32;
-    const int64_t lid = xcd_remap(blockIdx.x, nblocks);
-    const int64_t row = row0 + lid / segs_per_row;
-    const int64_t P0 = (lid % segs_per_row) * SEG_BLOCKS;  // first output block of the segment
-    const float *__restrict__ xrow = x + row * ldx;
-    const int64_t xb = 32 * P0 - 16 * (int64_t)m_hi;        // sample index of LDS unit 0
+    constexpr int PF = 2 * TW + 1;
+    const int64_t lid = xcd_remap(blockIdx.x, nwg);
+    const int64_t seg_first = lid * segs_per_wg;
+    const int64_t seg_end = min(nsegs, seg_first + segs_per_wg);
     const int units = SEG_BLOCKS * 4 + 2 * m_hi;
 
+    // Row pointer and sample index of LDS unit 0 of segment sg.
+    auto seg_base = [&](int64_t sg, const float *&xrow, int64_t &xb, int64_t &P0, int64_t &row) {
+        row = row0 + sg / segs_per_row;
+        P0 = (sg % segs_per_row) * SEG_BLOCKS;
+        xrow = x + row * ldx;
+        xb = 32 * P0 - 16 * (int64_t)m_hi;
+    };
+
+    f32x4 raw[PF][2];
+    auto prefetch = [&](int64_t sg, int tid) {
+        const float *xrow;
+        int64_t xb, P0, row;
+        seg_base(sg, xrow, xb, P0, row);
+#pragma unroll
+        for (int j = 0; j < PF; ++j)
+            load_unit(xrow, N, xb, tid + j * kThreads, units, vec_x, rev, raw[j]);
+    };
+    prefetch(seg_first, (int)threadIdx.x);
+
     // ---- tap tables: T_s[j] = split_s(b[C - j]), j in [0, L), two copies ----
-    const int C = 16 * (m_hi + 2 * TW + 1) + 32, L = C + 32 * TW + 32;
-    auto build_tables = [&]() {
+    // (built while the first segment's loads are in flight)
+    {
+        const int C = 16 * (m_hi + 2 * TW + 1) + 32, L = C + 32 * TW + 32;
         for (int q = threadIdx.x; q < 2 * (L + 1); q += kThreads) {
             const int c = q / (L + 1), pos = q - c * (L + 1), j = pos - c;
             const int k = C - j;
@@ -284,110 +365,85 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
                 *reinterpret_cast<unsigned short *>(lds + tab_offset + (2 * s2 + c) * copy_bytes +
                                                     2 * pos) = t[s2];
         }
-    };
-
-    // ---- stage: x[xb .. xb + 8*units) -> S split arrays in LDS ----
-    // Each lane issues the loads of kStageBatch units before splitting any of
-    // them, so their memory latencies overlap; the tap tables are built while
-    // the first batch is in flight.
-    for (int u0 = threadIdx.x; u0 < units; u0 += kStageBatch * kThreads) {
-        f32x4 raw[kStageBatch][2];
-#pragma unroll
-        for (int j = 0; j < kStageBatch; ++j) {
-            const int u = u0 + j * kThreads;
-            const int64_t g0 = xb + 8 * (int64_t)u;
-            if (u >= units) {
-                raw[j][0] = raw[j][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            } else if (vec_x && g0 >= 0 && g0 + 7 < N) {
-                if (!rev) {
-                    raw[j][0] =
-                        __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0));
-                    raw[j][1] =
-                        __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0 + 4));
-                } else {  // logical g0..g0+7 == physical N-1-g0 .. N-8-g0
-                    const f32x4 a = __builtin_nontemporal_load(
-                        reinterpret_cast<const f32x4 *>(xrow + (N - 4 - g0)));
-                    const f32x4 c = __builtin_nontemporal_load(
-                        reinterpret_cast<const f32x4 *>(xrow + (N - 8 - g0)));
-                    raw[j][0] = f32x4{a.w, a.z, a.y, a.x};
-                    raw[j][1] = f32x4{c.w, c.z, c.y, c.x};
-                }
-            } else {
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int64_t i = g0 + e;
-                    raw[j][e >> 2][e & 3] = (i >= 0 && i < N) ? xrow[rev ? N - 1 - i : i] : 0.0f;
-                }
-            }
-        }
-        if (u0 == (int)threadIdx.x) build_tables();
-#pragma unroll
-        for (int j = 0; j < kStageBatch; ++j) {
-            const int u = u0 + j * kThreads;
-            if (u >= units) break;
-            unsigned short t[S][8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const float v = raw[j][e >> 2][e & 3];
-                if (S == 3)
-                    split3(v, t[0][e], t[S > 1 ? 1 : 0][e], t[S > 2 ? 2 : 0][e]);
-                else
-                    t[0][e] = f16_bits((_Float16)v);
-            }
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                u32x4 w = {pack2(t[s][0], t[s][1]), pack2(t[s][2], t[s][3]),
-                           pack2(t[s][4], t[s][5]), pack2(t[s][6], t[s][7])};
-                *reinterpret_cast<u32x4 *>(lds + s * split_stride + unit_addr<TW>(u)) = w;
-            }
-        }
     }
 
-    __syncthreads();
+    const int C = 16 * (m_hi + 2 * TW + 1) + 32;
+    for (int64_t sg = seg_first; sg < seg_end; ++sg) {
+        // Lane-dependent values are recomputed in every iteration rather than
+        // hoisted out of the loop, where they would hold registers through
+        // the contraction.
+        int tid = (int)threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int wave = tid >> 6, lane = tid & 63;
+        const int i = lane & 31, g = lane >> 5;
+        const float *xrow;
+        int64_t xb, P0, row;
+        seg_base(sg, xrow, xb, P0, row);
 
-    // ---- contraction ----
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = lane & 31, g = lane >> 5;
-    Contraction<S, TW> ct;
-    ct.lds = lds;
-    ct.split_stride = split_stride;
-    ct.msteps = msteps;
-    ct.Q = msteps + 2 * TW - 2;
-    // D(n) = tile 0's B at half-step q = n - 2TW + 1: unit
-    // 4*(wave offset + TW*i) + 2*(m_hi - q) + g.
-    ct.ubase = 4 * (wave * 32 * TW + TW * i) + 2 * (m_hi + 2 * TW - 1) + g;
-    {
-        // A of half-step index sa (m = sa - 1) starts at entry
-        // j0 = C - 16*(sa - 1) - i + 8g, in the copy whose shift c makes j0 + c even.
-        const int c = (C - i) & 1;
+        // ---- stage: x[xb .. xb + 8*units) -> S split arrays in LDS ----
+        if (sg != seg_first) __syncthreads();  // the previous segment's MFMAs are done
 #pragma unroll
-        for (int s2 = 0; s2 < S; ++s2)
-            ct.abase[s2] = tab_offset + (2 * s2 + c) * copy_bytes + 2 * (C + 16 - i + 8 * g + c);
-    }
-    ct.run();
+        for (int j = 0; j < PF; ++j) {
+            const int u = tid + j * kThreads;
+            if (u < units) store_unit<S, TW>(lds, split_stride, u, raw[j]);
+        }
+        for (int u0 = tid + PF * kThreads; u0 < units; u0 += kRestBatch * kThreads) {
+            f32x4 rest[kRestBatch][2];
+#pragma unroll
+            for (int j = 0; j < kRestBatch; ++j)
+                load_unit(xrow, N, xb, u0 + j * kThreads, units, vec_x, rev, rest[j]);
+#pragma unroll
+            for (int j = 0; j < kRestBatch; ++j) {
+                const int u = u0 + j * kThreads;
+                if (u >= units) break;
+                store_unit<S, TW>(lds, split_stride, u, rest[j]);
+            }
+        }
+        __syncthreads();
+        if (sg + 1 < seg_end) prefetch(sg + 1, tid);
 
-    // ---- epilogue: tile t, lane (i, g): block P + t + TW*i, sample 8q + 4g + j ----
-    float *__restrict__ yrow = y + row * ldy;
+        // ---- contraction ----
+        Contraction<S, TW> ct;
+        ct.lds = lds;
+        ct.split_stride = split_stride;
+        ct.msteps = msteps;
+        ct.Q = msteps + 2 * TW - 2;
+        // D(n) = tile 0's B at half-step q = n - 2TW + 1: unit
+        // 4*(wave offset + TW*i) + 2*(m_hi - q) + g.
+        ct.ubase = 4 * (wave * 32 * TW + TW * i) + 2 * (m_hi + 2 * TW - 1) + g;
+        {
+            // A of half-step index sa (m = sa - 1) starts at entry
+            // j0 = C - 16*(sa - 1) - i + 8g, in the copy whose shift c makes j0 + c even.
+            const int c = (C - i) & 1;
 #pragma unroll
-    for (int t = 0; t < TW; ++t) {
-        const int64_t n0 = 32 * (P0 + wave * 32 * TW + t + TW * i) + 4 * g;
+            for (int s2 = 0; s2 < S; ++s2)
+                ct.abase[s2] = tab_offset + (2 * s2 + c) * copy_bytes + 2 * (C + 16 - i + 8 * g + c);
+        }
+        ct.run();
+
+        // ---- epilogue: tile t, lane (i, g): block P + t + TW*i, sample 8q + 4g + j ----
+        float *__restrict__ yrow = y + row * ldy;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            f32x4 v;
-            v.x = ct.acc0[t][4 * q + 0] + ct.acc1[t][4 * q + 0];
-            v.y = ct.acc0[t][4 * q + 1] + ct.acc1[t][4 * q + 1];
-            v.z = ct.acc0[t][4 * q + 2] + ct.acc1[t][4 * q + 2];
-            v.w = ct.acc0[t][4 * q + 3] + ct.acc1[t][4 * q + 3];
-            const int64_t n = n0 + 8 * q;
-            if (vec_y && n + 3 < N) {
-                if (!rev)
-                    *reinterpret_cast<f32x4 *>(yrow + n) = v;
-                else
-                    *reinterpret_cast<f32x4 *>(yrow + (N - 4 - n)) = f32x4{v.w, v.z, v.y, v.x};
-            } else {
+        for (int t = 0; t < TW; ++t) {
+            const int64_t n0 = 32 * (P0 + wave * 32 * TW + t + TW * i) + 4 * g;
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (n + j < N) yrow[rev ? N - 1 - (n + j) : n + j] = v[j];
+            for (int q = 0; q < 4; ++q) {
+                f32x4 v;
+                v.x = ct.acc0[t][4 * q + 0] + ct.acc1[t][4 * q + 0];
+                v.y = ct.acc0[t][4 * q + 1] + ct.acc1[t][4 * q + 1];
+                v.z = ct.acc0[t][4 * q + 2] + ct.acc1[t][4 * q + 2];
+                v.w = ct.acc0[t][4 * q + 3] + ct.acc1[t][4 * q + 3];
+                const int64_t n = n0 + 8 * q;
+                if (vec_y && n + 3 < N) {
+                    if (!rev)
+                        *reinterpret_cast<f32x4 *>(yrow + n) = v;
+                    else
+                        *reinterpret_cast<f32x4 *>(yrow + (N - 4 - n)) = f32x4{v.w, v.z, v.y, v.x};
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (n + j < N) yrow[rev ? N - 1 - (n + j) : n + j] = v[j];
+                }
             }
         }
     }
@@ -408,7 +464,7 @@ size_t mfma_lds_bytes(int K)
 
 template <int S, int TW>
 int launch_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
-                float *y, int64_t ldy, bool rev, hipStream_t stream)
+                float *y, int64_t ldy, bool rev, int segs_per_wg, hipStream_t stream)
 {
     constexpr int SEG_BLOCKS = kWaves * TW * 32;
     const int m_hi = mfma_m_hi(K), msteps = mfma_steps(K);
@@ -432,10 +488,12 @@ int launch_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, i
     const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / segs);
     for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
         const int64_t rows = std::min(rows_per_launch, B - r0);
-        const int64_t nb = rows * segs;
-        hipLaunchKernelGGL((fir_mfma_kernel<S, TW>), dim3((unsigned)nb), dim3(kThreads), lds,
-                           stream, x, ldx, N, y, ldy, b, K, msteps, m_hi, r0, (int)segs, nb,
-                           split_stride, tab_offset, copy_bytes, vec_x, vec_y, (int)rev);
+        const int64_t nsegs = rows * segs;
+        const int64_t nwg = (nsegs + segs_per_wg - 1) / segs_per_wg;
+        hipLaunchKernelGGL((fir_mfma_kernel<S, TW>), dim3((unsigned)nwg), dim3(kThreads), lds,
+                           stream, x, ldx, N, y, ldy, b, K, msteps, m_hi, r0, (int)segs, nsegs,
+                           segs_per_wg, nwg, split_stride, tab_offset, copy_bytes, vec_x, vec_y,
+                           (int)rev);
         SCIR_TRY(check_launch("fir_mfma_kernel"));
     }
     return SCIR_HIP_OK;
@@ -443,7 +501,7 @@ int launch_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, i
 
 template <int S>
 int run_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N, float *y,
-             int64_t ldy, bool rev, hipStream_t stream)
+             int64_t ldy, bool rev, hipStream_t stream, const DeviceState *st)
 {
     const TuneChoice tc = tune_lookup(K, N);
     int tw = S == 3 ? tc.mfma_tiles : tc.mfma_tiles_fp16;
@@ -454,10 +512,18 @@ int run_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int6
     if (tw >= 2 && mfma_lds_bytes<S, 2>(K) > (size_t)kMaxLds) tw = 1;
     if (mfma_lds_bytes<S, 1>(K) > (size_t)kMaxLds)
         return fail(SCIR_HIP_ERR_UNSUPPORTED, "K=%d needs more LDS than the MFMA path has", K);
+    // Segments per workgroup: as tuned, but (unless forced) never so many
+    // that fewer than four workgroups per CU remain.
+    const int64_t seg_len = 32 * (int64_t)kWaves * tw * 32;
+    const int64_t total = B * ((N + seg_len - 1) / seg_len);
+    const int64_t cap = tc.mfma_segs_forced
+                            ? total
+                            : std::max<int64_t>(1, total / (4 * (int64_t)std::max(1, st->cu_count)));
+    const int spw = (int)std::max<int64_t>(1, std::min<int64_t>(tc.mfma_segs, cap));
     switch (tw) {
-    case 1: return launch_mfma<S, 1>(x, ldx, b, K, B, N, y, ldy, rev, stream);
-    case 4: return launch_mfma<S, 4>(x, ldx, b, K, B, N, y, ldy, rev, stream);
-    default: return launch_mfma<S, 2>(x, ldx, b, K, B, N, y, ldy, rev, stream);
+    case 1: return launch_mfma<S, 1>(x, ldx, b, K, B, N, y, ldy, rev, spw, stream);
+    case 4: return launch_mfma<S, 4>(x, ldx, b, K, B, N, y, ldy, rev, spw, stream);
+    default: return launch_mfma<S, 2>(x, ldx, b, K, B, N, y, ldy, rev, spw, stream);
     }
 }
 
@@ -465,14 +531,14 @@ int run_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int6
 
 int fir_mfma_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
                  float *y, int64_t ldy, int precision, bool rev, hipStream_t stream,
-                 DeviceState *)
+                 DeviceState *st)
 {
     if (B == 0 || N == 0) return SCIR_HIP_OK;
     if (K == 0) return fill_zero_2d(y, ldy, B, N, stream);
     if (precision == SCIR_HIP_PREC_BF16X3)
-        return run_mfma<3>(x, ldx, b, K, B, N, y, ldy, rev, stream);
+        return run_mfma<3>(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
     if (precision == SCIR_HIP_PREC_FP16)
-        return run_mfma<1>(x, ldx, b, K, B, N, y, ldy, rev, stream);
+        return run_mfma<1>(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
     return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "precision %d is not an MFMA mode", precision);
 }
 

@@ -104,6 +104,9 @@ class Graph:
         if exc_type is None:
             _lib.check(status)
             self._h = h
+        elif status == _lib.OK and h:
+            # The capture body raised: drop whatever was captured.
+            _lib.lib().scir_hip_graph_destroy(h)
         return False
 
     def launch(self):

@@ -8,7 +8,7 @@ from scir_amd import _lib
 TABLE = """{"arch": "gfx950", "auto_mfma_min_k": 128,
  "entries": [
   {"k_lo": 1, "k_hi": 16, "n_lo": 0, "n_hi": 8192, "direct_r": 8, "direct_loader": 1, "direct_seg_tiles": 2, "mfma_tiles": 2, "mfma_tiles_fp16": 4},
-  {"k_lo": 17, "k_hi": 64, "n_lo": 8193, "n_hi": 4611686018427387904, "direct_r": 16, "direct_loader": 0, "direct_seg_tiles": 1, "mfma_tiles": 1, "mfma_tiles_fp16": 2}]}
+  {"k_lo": 17, "k_hi": 64, "n_lo": 8193, "n_hi": 4611686018427387904, "direct_r": 16, "direct_loader": 0, "direct_seg_tiles": 1, "mfma_tiles": 1, "mfma_tiles_fp16": 2, "mfma_segs": 4}]}
 """
 
 
@@ -34,6 +34,9 @@ def test_overrides_validated():
     L = _lib.lib()
     assert L.scir_hip_tune_set(b"direct.r", 12) == _lib.ERR_INVALID_ARGUMENT
     assert L.scir_hip_tune_set(b"mfma.tiles", 3) == _lib.ERR_INVALID_ARGUMENT
+    assert L.scir_hip_tune_set(b"mfma.segs", 0) == _lib.ERR_INVALID_ARGUMENT
+    assert L.scir_hip_tune_set(b"mfma.segs", 8) == _lib.OK
+    assert L.scir_hip_tune_set(b"mfma.segs", -1) == _lib.OK
     assert L.scir_hip_tune_set(b"no.such.key", 1) == _lib.ERR_INVALID_ARGUMENT
     assert L.scir_hip_tune_set(b"direct.r", 16) == _lib.OK
     assert L.scir_hip_tune_set(b"direct.r", -1) == _lib.OK

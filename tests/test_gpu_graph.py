@@ -1,0 +1,78 @@
+"""HIP-graph capture of the dense device tier and the adjoints.
+
+The library's kernels take every buffer as an argument and clear memory
+with kernels (never hipMemsetAsync), so a captured sequence replays the
+eager results exactly.  Each replay here follows eager work and a
+synchronisation, the inputs change between replays (the graph reads them
+in place), and every replay is compared bit for bit with an eager call on
+the same inputs.  A buffer that would have to grow while a stream captures
+is refused with SCIR_HIP_ERR_CAPTURE rather than allocated.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rng
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision,K", [("f32", 33), ("f32_exact", 33), ("bf16x3", 512),
+                                         ("fp16", 512), ("auto", 129)])
+def test_dense_graph_replays_match_eager(hip, precision, K):
+    import torch
+    from scir_amd import device
+    from scir_amd.ragged import Graph
+    r = rng(K + len(precision))
+    B, N = 6, 50000
+    x = torch.tensor(r.standard_normal((B, N)).astype(np.float32), device="cuda")
+    g = torch.tensor(r.standard_normal((B, N)).astype(np.float32), device="cuda")
+    b = torch.tensor(r.standard_normal(K).astype(np.float32), device="cuda")
+    s = torch.cuda.Stream()
+    y, dx, db = torch.empty_like(x), torch.empty_like(x), torch.empty(K, device="cuda")
+    # Eager calls on the capture stream first: the per-stream dL/db partials
+    # are allocated here, outside the capture.
+    with torch.cuda.stream(s):
+        device.fir(x, b, precision, out=y)
+        device.vjp_dx(g, b, precision, out=dx)
+        device.vjp_db(g, x, K, out=db)
+    s.synchronize()
+    torch.cuda.synchronize()
+    with Graph(s) as gr:
+        device.fir(x, b, precision, out=y)
+        device.vjp_dx(g, b, precision, out=dx)
+        device.vjp_db(g, x, K, out=db)
+    for rep in range(3):
+        x.copy_(torch.tensor(r.standard_normal((B, N)).astype(np.float32), device="cuda"))
+        g.mul_(-0.5)
+        ye = device.fir(x, b, precision)
+        dxe = device.vjp_dx(g, b, precision)
+        dbe = device.vjp_db(g, x, K)
+        for t in (y, dx, db):
+            t.fill_(float("nan"))
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            gr.launch()
+        s.synchronize()
+        assert torch.equal(y, ye), f"{precision}: forward replay {rep} differs"
+        assert torch.equal(dx, dxe), f"{precision}: dL/dx replay {rep} differs"
+        assert torch.equal(db, dbe), f"{precision}: dL/db replay {rep} differs"
+    gr.close()
+
+
+def test_growth_during_capture_is_refused(hip):
+    """dL/db on a fresh stream needs its partials buffer: inside a capture
+    that is an error (SCIR_HIP_ERR_CAPTURE), not a hidden allocation."""
+    import torch
+    from scir_amd import _lib, device
+    from scir_amd.ragged import Graph
+    x = torch.ones((2, 70000), device="cuda")
+    device.vjp_db(x, x, 64)  # eager, on the current stream
+    s = torch.cuda.Stream()  # fresh stream: no partials buffer yet
+    torch.cuda.synchronize()
+    with pytest.raises(_lib.HipError) as ei:
+        with Graph(s):
+            device.vjp_db(x, x, 64)
+    assert "captur" in str(ei.value).lower()
+    torch.cuda.synchronize()

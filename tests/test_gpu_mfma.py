@@ -123,3 +123,41 @@ def test_full_size_long_taps_rel_l2(hip, oracle, K):
     print(f"K={K}: rel-L2 bf16x3 {rel3:.3g}, fp16 {rel16:.3g}")
     assert rel3 <= 1e-6
     assert np.all(np.abs(y3[rows].cpu().numpy() - y64) <= oracle.fir_error_bound(xs, b_np))
+
+
+@pytest.fixture(params=[2, 5, 64], ids=lambda s: f"segs{s}")
+def segs(request, hip):
+    L = hip.lib()
+    hip.check(L.scir_hip_tune_set(b"mfma.segs", request.param))
+    yield request.param
+    hip.check(L.scir_hip_tune_set(b"mfma.segs", -1))
+
+
+@pytest.mark.parametrize("K", [17, 512, 2048, 4097])
+def test_segments_per_workgroup(hip, oracle, tiles, segs, K):
+    """One workgroup walking several row segments (next segment's samples
+    prefetched during the current one's MFMAs): segments that cross row
+    boundaries, a partial last segment, and K > 2049 (history longer than
+    the prefetched part), forward and anti-causal (dL/dx)."""
+    import torch
+    from scir_amd import device
+    r = rng(700 + K + segs)
+    B, N = 7, 20000
+    x = r.standard_normal((B, N)).astype(np.float32)
+    x[3, ::37] *= 1000.0
+    b = r.standard_normal(K).astype(np.float32)
+    y64 = oracle.lfilter_f64(x, b)
+    for prec in ("bf16x3", "fp16"):
+        y = host_fir(hip, x, b, prec)
+        if prec == "bf16x3":
+            bound = oracle.fir_error_bound(x, b)
+        else:
+            bound = (2.0 ** -10 + (K + 2) * EPS) * oracle.abs_bound(x, b)
+        assert np.all(np.abs(y - y64) <= bound), f"{prec} K={K} segs={segs} tiles={tiles}"
+    w = r.standard_normal((B, N)).astype(np.float32)
+    dx = device.vjp_dx(torch.tensor(w, device="cuda"), torch.tensor(b, device="cuda"), "bf16x3")
+    torch.cuda.synchronize()
+    dx64, _ = oracle.vjp_f64(w, x, b)
+    dxa, _ = oracle.vjp_f64(np.abs(w), np.abs(x), np.abs(b))
+    assert np.all(np.abs(dx.cpu().numpy() - dx64) <= (K + 1) * EPS * dxa)
+
