@@ -14,7 +14,12 @@ after the timed region on the same inputs:
                                       and rel-L2 vs the FP64 oracle
   bandwidth_workload                  x [4096, 65536], 33-tap low-pass, FP32
                                       direct: GB/s against the 8 TB/s HBM peak
-  roofline                            the dominant kernel of the headline
+  roofline                            the dominant kernel of the headline:
+                                      algorithmic TFLOP/s against 2.5 PFLOP/s,
+                                      the issued MFMA rate (mfma_issued_frac),
+                                      and from the committed rocprofv3
+                                      counters: MFMA-busy fraction of SIMD
+                                      cycles and the shader clock
   cpu_baseline                        the oracle's restatement of the
                                       reference CPU loop, one thread, on whole
                                       rows of the same workload (~10 s)
@@ -73,13 +78,18 @@ def describe_taps(kind, K):
     return f"{K}-tap Kaiser(8) band-pass 0.1-0.3 of Nyquist"
 
 
-def load_traffic(tag):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+def load_profile(tag, key):
+    """A figure from a committed rocprofv3 --pmc summary under profiles/."""
     path = os.path.join(ROOT, "profiles", f"{tag}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        return json.load(f).get(key)
+
+
+def load_traffic(tag):
+    """HBM bytes per launch of the kernel (FETCH_SIZE and WRITE_SIZE runs)."""
+    return load_profile(tag, "hbm_bytes_per_launch")
 
 
 def timed(fn, steps, warmup, stream):
@@ -168,7 +178,9 @@ def roofline_for(wl, name, prec, avg_launch_s):
                     frac=round(ach / BF16_PEAK_TFLOPS, 4),
                     traffic=load_traffic(f"fir_mfma_{prec}_{name}_pmc"),
                     mfma_issued_tflops=round(issued / avg_launch_s / 1e12, 1),
-                    mfma_issued_frac=round(issued / avg_launch_s / 1e12 / BF16_PEAK_TFLOPS, 4))
+                    mfma_issued_frac=round(issued / avg_launch_s / 1e12 / BF16_PEAK_TFLOPS, 4),
+                    mfma_busy_frac=load_profile(f"fir_mfma_{prec}_{name}_util", "mfma_busy_frac"),
+                    sclk_ghz=load_profile(f"fir_mfma_{prec}_{name}_util", "sclk_ghz"))
     flops = 2.0 * wl["K"] * samples
     if flops / (8.0 * samples) < 78.6 / 8.0:  # below the FP32-vector ridge: HBM-bound
         alg = 8.0 * samples
@@ -218,7 +230,8 @@ def run_dense(args):
         if name != "direct_k33":
             wl2, x2, b2, _ = dense_setup("direct_k33")
             y2 = torch.empty_like(x2)
-            ms2, per2, _ = timed(lambda: device.fir(x2, b2, "f32", out=y2), 10, 3, stream)
+            # 60 untimed launches (~26 ms): the clock settles after a load step.
+            ms2, per2, _ = timed(lambda: device.fir(x2, b2, "f32", out=y2), 20, 60, stream)
             r2 = roofline_for(wl2, "direct_k33", "f32", float(np.mean(per2)) * 1e-3)
             out["bandwidth_workload"] = dict(
                 workload="direct_k33", rows=wl2["B"], samples_per_row=wl2["N"], taps=33,
@@ -335,7 +348,7 @@ def main():
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default=HEADLINE,
                     choices=sorted(DENSE) + ["ragged", "vjp_k255", "copy_1gib"])
     ap.add_argument("--precision", default=None)

@@ -105,7 +105,7 @@ struct TuneChoice {
     int mfma_tiles_fp16 = 4;  // the same for the FP16 MFMA kernel
     int mfma_segs = 16;       // row segments per workgroup of the MFMA kernel
     bool mfma_segs_forced = false;  // set through scir_hip_tune_set: no grid-size cap
-    int auto_mfma_min_k = 96; // AUTO uses BF16x3 MFMA from this K up
+    int auto_mfma_min_k = 56; // AUTO uses BF16x3 MFMA from this K up
 };
 TuneChoice tune_lookup(int K, int64_t N);
 
