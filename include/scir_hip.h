@@ -177,7 +177,11 @@ SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
  * the table; value < 0 removes the override.  Keys:
  *   "direct.r"        outputs per lane of the direct kernel (8 or 16)
  *   "direct.loader"   0 = one tile per workgroup, 1 = pipelined row segments
+ *                     (register prefetch), 2 = pipelined row segments copied
+ *                     into LDS by DMA two tiles ahead (forward, 16-byte aligned
+ *                     rows with N % 4 == 0; other calls use 1)
  *   "direct.seg_tiles" tiles per workgroup of the pipelined direct kernel
+ *   "direct.dma_bufs" LDS tile buffers of loader 2 (2, 3 or 4: 1 to 3 tiles in flight)
  *   "mfma.tiles"      32x32 output tiles per wave of the MFMA kernel (1, 2, 4)
  *   "mfma.segs"       row segments per workgroup of the MFMA kernel (>= 1); the
  *                     next segment's samples load while the current one computes

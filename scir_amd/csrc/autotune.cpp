@@ -1,7 +1,9 @@
 // Tuning table and the autotuner.
 //
 // The table maps (K bucket, N bucket) to launch shapes:
-//   direct_r / direct_loader / direct_seg_tiles   FP32 direct kernel
+//   direct_r / direct_loader / direct_seg_tiles   FP32 direct kernel (loader 0: one
+//                                                 tile per workgroup, 1: register
+//                                                 prefetch, 2: LDS DMA)
 //   mfma_tiles / mfma_tiles_fp16                  MFMA kernel tiles per wave
 //   mfma_segs                                     MFMA row segments per workgroup
 // plus one global AUTO cut-over K (F32 direct below, BF16x3 MFMA from there).
@@ -39,7 +41,8 @@ std::vector<Entry> g_table;
 int g_auto_min_k = -1;  // from the table; -1 = built-in default
 // Overrides set through scir_hip_tune_set (-1 = unset).
 int g_over_direct_r = -1, g_over_direct_loader = -1, g_over_direct_seg = -1,
-    g_over_mfma_tiles = -1, g_over_auto_min_k = -1, g_over_mfma_segs = -1;
+    g_over_mfma_tiles = -1, g_over_auto_min_k = -1, g_over_mfma_segs = -1,
+    g_over_dma_bufs = -1;
 
 // Buckets searched by the autotuner.
 const int kKBuckets[][2] = {{1, 16}, {17, 64}, {65, 256}, {257, 1017}, {1018, 4096}};
@@ -63,6 +66,7 @@ TuneChoice tune_lookup(int K, int64_t N)
     if (g_over_direct_r >= 0) c.direct_r = g_over_direct_r;
     if (g_over_direct_loader >= 0) c.direct_loader = g_over_direct_loader;
     if (g_over_direct_seg >= 0) c.direct_seg_tiles = g_over_direct_seg;
+    if (g_over_dma_bufs >= 0) c.direct_dma_bufs = g_over_dma_bufs;
     if (g_over_mfma_tiles >= 0) c.mfma_tiles = c.mfma_tiles_fp16 = g_over_mfma_tiles;
     if (g_over_auto_min_k >= 0) c.auto_mfma_min_k = g_over_auto_min_k;
     if (g_over_mfma_segs >= 0) {
@@ -170,8 +174,12 @@ extern "C" int scir_hip_tune_set(const char *key, int32_t value)
             return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "direct.r must be 8 or 16");
         g_over_direct_r = v;
     } else if (!std::strcmp(key, "direct.loader")) {
-        if (v > 1) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "direct.loader must be 0 or 1");
+        if (v > 2) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "direct.loader must be 0, 1 or 2");
         g_over_direct_loader = v;
+    } else if (!std::strcmp(key, "direct.dma_bufs")) {
+        if (v >= 0 && (v < 2 || v > 4))
+            return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "direct.dma_bufs must be 2, 3 or 4");
+        g_over_dma_bufs = v;
     } else if (!std::strcmp(key, "direct.seg_tiles")) {
         if (v == 0) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "direct.seg_tiles must be >= 1");
         g_over_direct_seg = v;
@@ -225,7 +233,7 @@ extern "C" int scir_hip_autotune_load(const char *cache_path)
         e.n_lo = v[2];
         e.n_hi = v[3];
         e.c.direct_r = (v[4] == 16) ? 16 : 8;
-        e.c.direct_loader = v[5] ? 1 : 0;
+        e.c.direct_loader = (v[5] >= 0 && v[5] <= 2) ? (int)v[5] : 1;
         e.c.direct_seg_tiles = (int)std::max<int64_t>(1, v[6]);
         e.c.mfma_tiles = (v[7] == 1 || v[7] == 4) ? (int)v[7] : 2;
         e.c.mfma_tiles_fp16 = (v[8] == 1 || v[8] == 2) ? (int)v[8] : 4;
@@ -314,7 +322,7 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
             e.n_lo = nb[0];
             e.n_hi = nb[1];
             float best = 1e30f;
-            const int rs[2] = {8, 16}, loaders[2] = {0, 1}, segs[3] = {1, 2, 8};
+            const int rs[2] = {8, 16}, loaders[3] = {0, 1, 2}, segs[4] = {1, 2, 4, 8};
             for (int r : rs)
                 for (int ld : loaders)
                     for (int sg : segs) {

@@ -99,8 +99,9 @@ int fill_zero_2d(float *y, int64_t ldy, int64_t B, int64_t N, hipStream_t stream
 // Autotuned choices (defaults until a table is loaded or a search ran).
 struct TuneChoice {
     int direct_r = 8;         // outputs per lane in the direct kernel
-    int direct_loader = 1;    // 0: one tile per workgroup, 1: pipelined row segments
-    int direct_seg_tiles = 8; // tiles per workgroup of the pipelined variant
+    int direct_loader = 2;    // 0: one tile per workgroup, 1: register prefetch, 2: LDS DMA
+    int direct_seg_tiles = 4; // tiles per workgroup of the pipelined variants
+    int direct_dma_bufs = 2;  // LDS buffers of the DMA loader (tiles in flight + 1), 2..4
     int mfma_tiles = 2;       // 32x32 output tiles per wave, BF16x3 MFMA kernel
     int mfma_tiles_fp16 = 4;  // the same for the FP16 MFMA kernel
     int mfma_segs = 16;       // row segments per workgroup of the MFMA kernel

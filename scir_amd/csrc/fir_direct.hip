@@ -122,6 +122,116 @@ __global__ __launch_bounds__(kThreads) void fir_direct_pipe_kernel(
     }
 }
 
+// LDS-DMA variant (forward direction, 16-byte aligned rows, N % 4 == 0):
+// tiles are copied from HBM straight into LDS with buffer_load_dwordx4 ... lds,
+// NB-1 tiles ahead (NB LDS buffers), so no registers or VALU instructions
+// carry the staged samples and each workgroup keeps two tiles of loads in
+// flight.  The row is addressed through a buffer resource of N*4 bytes:
+// chunks before the row start (negative offsets wrap past the record count)
+// and past its end read as zero, and output stores past the end are dropped,
+// so every wave issues the same number of memory instructions per tile and
+// its waits can be counted exactly.
+//
+// Wave w copies the LDS chunk groups q = w, w+4, ... (64 chunks of 16 bytes,
+// one instruction each) with 64*q < count4; lane l of group q fills tile
+// chunk 64q + l (the linear LDS layout of direct_tiles.hpp).
+__device__ __forceinline__ void dma_chunk16(u32x4 rsrc, unsigned voffset, unsigned lds_byte)
+{
+    // M0 (the LDS destination of the wave) is written by the compiler just
+    // before this block; s_nop provides the wait state an LDS DMA needs after
+    // an M0 write.
+    asm volatile("s_nop 0\n\tbuffer_load_dwordx4 %0, %1, 0 offen lds"
+                 :
+                 : "v"(voffset), "s"(rsrc), "{m0}"(lds_byte)
+                 : "memory");
+}
+
+// Buffer resource of a row of N floats (raw: stride 0, range N*4 bytes).
+__device__ __forceinline__ u32x4 row_rsrc(const float *row, int64_t N)
+{
+    const uint64_t base = reinterpret_cast<uint64_t>(row);
+    return u32x4{(unsigned)base, (unsigned)(base >> 32) & 0xffffu, (unsigned)(N * 4), 0x00020000u};
+}
+
+__device__ __forceinline__ void store_dwordx4(__amdgpu_buffer_rsrc_t rsrc, unsigned voffset,
+                                              f32x4 v)
+{
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), rsrc, voffset, 0, 0);
+}
+
+// s_waitcnt vmcnt(n) for a wave-uniform n, leaving the other counters
+// unconstrained; n > 15 waits for everything (never too early).
+__device__ __forceinline__ void wait_vmcnt(int n)
+{
+#define SCIR_VMW(k) \
+    case k: __builtin_amdgcn_s_waitcnt((k) | (7 << 4) | (0xf << 8)); break;
+    switch (n) {
+        SCIR_VMW(1) SCIR_VMW(2) SCIR_VMW(3) SCIR_VMW(4) SCIR_VMW(5) SCIR_VMW(6) SCIR_VMW(7)
+        SCIR_VMW(8) SCIR_VMW(9) SCIR_VMW(10) SCIR_VMW(11) SCIR_VMW(12) SCIR_VMW(13)
+        SCIR_VMW(14) SCIR_VMW(15)
+        default: __builtin_amdgcn_s_waitcnt(0 | (7 << 4) | (0xf << 8)); break;
+    }
+#undef SCIR_VMW
+}
+
+template <int R, bool EXACT, int NB>
+__global__ __launch_bounds__(kThreads) void fir_direct_dma_kernel(
+    const float *__restrict__ x, int64_t ldx, const float *__restrict__ b, int K, int64_t N,
+    float *__restrict__ y, int64_t ldy, int64_t row0, int segs_per_row, int tiles_per_seg,
+    int64_t tiles_per_row, int buf_chunks)
+{
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    constexpr int T = kThreads * R;
+    constexpr int S = R / 4;  // output stores per lane and tile
+    const int H = (K + 6) & ~3;
+    const int count4 = (T + H + 4) / 4;  // tile chunks, <= buf_chunks (checked by the launcher)
+    const int64_t row = row0 + blockIdx.x / segs_per_row;
+    const int64_t tfirst = (int64_t)(blockIdx.x % segs_per_row) * tiles_per_seg;
+    const int ntiles = (int)min<int64_t>((int64_t)tiles_per_seg, tiles_per_row - tfirst);
+    const u32x4 xr = row_rsrc(x + row * ldx, N);
+    const __amdgpu_buffer_rsrc_t yr =
+        __builtin_amdgcn_make_buffer_rsrc(y + row * ldy, 0, (int)(N * 4), 0x00020000);
+    // The wave index is uniform: keep it (and what derives from it) scalar.
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int lane_base = threadIdx.x * R;
+    // DMA instructions of this wave per tile.
+    const int nd = (count4 + 255 - 64 * wave) / 256;
+    const unsigned lds0 =
+        (unsigned)reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) float *)lds);
+
+    auto issue = [&](int64_t tile, int slot) {
+        const int64_t g0 = tile * T - H;  // sample of tile chunk 0
+        for (int d = 0; d < nd; ++d) {
+            const int q = wave + 4 * d;
+            const int c = 64 * q + lane;
+            // Chunks past the tile read as zero (offset beyond the record count).
+            const unsigned voff = c < count4 ? (unsigned)(4 * (g0 + 4 * (int64_t)c)) : 0x80000000u;
+            dma_chunk16(xr, voff, lds0 + 16u * (unsigned)(slot * buf_chunks + 64 * q));
+        }
+    };
+
+    for (int t = 0; t < NB - 1 && t < ntiles; ++t) issue(tfirst + t, t);
+    for (int it = 0; it < ntiles; ++it) {
+        // VMEM instructions this wave issued after tile it's copies: the
+        // copies of the tiles it+1 .. it+NB-2 and the stores of up to NB-1
+        // earlier tiles.  Memory instructions of a wave complete in order.
+        wait_vmcnt(nd * max(0, min(NB - 2, ntiles - 1 - it)) + S * min(it, NB - 1));
+        __syncthreads();
+        if (it + NB - 1 < ntiles) issue(tfirst + it + NB - 1, (it + NB - 1) % NB);
+        const float *buf = lds + (it % NB) * buf_chunks * 4;
+        float acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = 0.0f;
+        taps_from_lds<R, EXACT, kLinear>(acc, buf, H, lane_base, K, b);
+        const int64_t i0 = (tfirst + it) * T + lane_base;
+#pragma unroll
+        for (int r = 0; r < R; r += 4)
+            store_dwordx4(yr, (unsigned)(4 * (i0 + r)),
+                          f32x4{acc[r], acc[r + 1], acc[r + 2], acc[r + 3]});
+    }
+}
+
 constexpr int kPipeMaxK = 1017;  // (K + 6) & ~3 <= 1020, so T + H + 4 <= T + 1024
 
 int align_flags(const float *p, int64_t ld) {
@@ -185,6 +295,31 @@ int launch_direct_pipe(const float *x, int64_t ldx, const float *b, int K, int64
     return SCIR_HIP_OK;
 }
 
+template <int R, bool EXACT, int NB>
+int launch_direct_dma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
+                      float *y, int64_t ldy, int seg_tiles, hipStream_t stream)
+{
+    constexpr int T = kThreads * R;
+    const int H = (K + 6) & ~3;
+    const int count4 = (T + H + 4) / 4;
+    const int buf_chunks = (count4 + 63) / 64 * 64;
+    if (K > kPipeMaxK || N % 4 != 0 || N >= ((int64_t)1 << 29))
+        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "LDS-DMA direct kernel: unsupported shape");
+    const size_t lds = NB * (size_t)buf_chunks * 16;
+    const int64_t tiles = (N + T - 1) / T;
+    const int64_t tps = std::max<int64_t>(1, std::min<int64_t>(seg_tiles, tiles));
+    const int64_t segs = (tiles + tps - 1) / tps;
+    const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / segs);
+    for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
+        const int64_t rows = std::min(rows_per_launch, B - r0);
+        hipLaunchKernelGGL((fir_direct_dma_kernel<R, EXACT, NB>), dim3((unsigned)(rows * segs)),
+                           dim3(kThreads), lds, stream, x, ldx, b, K, N, y, ldy, r0, (int)segs,
+                           (int)tps, tiles, buf_chunks);
+        SCIR_TRY(check_launch("fir_direct_dma_kernel"));
+    }
+    return SCIR_HIP_OK;
+}
+
 }  // namespace
 
 int fir_direct_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
@@ -194,7 +329,21 @@ int fir_direct_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B
     if (K == 0) return fill_zero_2d(y, ldy, B, N, stream);
     const TuneChoice tc = tune_lookup(K, N);
     const bool r8 = (tc.direct_r == 8) || N <= 2048;
-    if (tc.direct_loader == 1 && K <= kPipeMaxK) {
+    // LDS-DMA loader: forward rows, 16-byte aligned, N % 4 == 0, N < 2^29.
+    if (tc.direct_loader == 2 && K <= kPipeMaxK && !rev && N % 4 == 0 &&
+        N < ((int64_t)1 << 29) && align_flags(x, ldx) && align_flags(y, ldy)) {
+        const int st = tc.direct_seg_tiles;
+        const int nb = tc.direct_dma_bufs;
+#define SCIR_DMA(RR, EX, NBV) launch_direct_dma<RR, EX, NBV>(x, ldx, b, K, B, N, y, ldy, st, stream)
+        if (exact) {
+            if (!r8) return nb == 2 ? SCIR_DMA(16, true, 2) : SCIR_DMA(16, true, 3);
+            return nb == 2 ? SCIR_DMA(8, true, 2) : nb == 4 ? SCIR_DMA(8, true, 4) : SCIR_DMA(8, true, 3);
+        }
+        if (!r8) return nb == 2 ? SCIR_DMA(16, false, 2) : SCIR_DMA(16, false, 3);
+        return nb == 2 ? SCIR_DMA(8, false, 2) : nb == 4 ? SCIR_DMA(8, false, 4) : SCIR_DMA(8, false, 3);
+#undef SCIR_DMA
+    }
+    if (tc.direct_loader >= 1 && K <= kPipeMaxK) {
         const int st = tc.direct_seg_tiles;
         if (exact)
             return r8 ? launch_direct_pipe<8, true>(x, ldx, b, K, B, N, y, ldy, st, rev, stream)

@@ -35,6 +35,10 @@ def test_overrides_validated():
     assert L.scir_hip_tune_set(b"direct.r", 12) == _lib.ERR_INVALID_ARGUMENT
     assert L.scir_hip_tune_set(b"mfma.tiles", 3) == _lib.ERR_INVALID_ARGUMENT
     assert L.scir_hip_tune_set(b"mfma.segs", 0) == _lib.ERR_INVALID_ARGUMENT
+    assert L.scir_hip_tune_set(b"direct.dma_bufs", 5) == _lib.ERR_INVALID_ARGUMENT
+    assert L.scir_hip_tune_set(b"direct.loader", 3) == _lib.ERR_INVALID_ARGUMENT
+    assert L.scir_hip_tune_set(b"direct.dma_bufs", 2) == _lib.OK
+    assert L.scir_hip_tune_set(b"direct.dma_bufs", -1) == _lib.OK
     assert L.scir_hip_tune_set(b"mfma.segs", 8) == _lib.OK
     assert L.scir_hip_tune_set(b"mfma.segs", -1) == _lib.OK
     assert L.scir_hip_tune_set(b"no.such.key", 1) == _lib.ERR_INVALID_ARGUMENT

@@ -32,18 +32,21 @@ def host_fir(hip, x, b, precision):
     return y
 
 
-@pytest.fixture(params=[(0, 16, 1), (0, 8, 1), (1, 16, 8), (1, 8, 3), (1, 8, 1)],
-                ids=lambda p: f"loader{p[0]}-r{p[1]}-seg{p[2]}")
+@pytest.fixture(params=[(0, 16, 1, 3), (0, 8, 1, 3), (1, 16, 8, 3), (1, 8, 3, 3), (1, 8, 1, 3),
+                        (2, 8, 8, 3), (2, 8, 1, 3), (2, 8, 2, 2), (2, 8, 5, 4), (2, 16, 3, 3)],
+                ids=lambda p: f"loader{p[0]}-r{p[1]}-seg{p[2]}-bufs{p[3]}")
 def variant(request, hip):
     """Each direct-kernel variant: one tile per workgroup or pipelined row
-    segments of `seg` tiles, 8 or 16 outputs per lane."""
-    loader, r, seg = request.param
+    segments of `seg` tiles (register prefetch, or LDS DMA with `bufs` tile
+    buffers), 8 or 16 outputs per lane."""
+    loader, r, seg, bufs = request.param
     L = hip.lib()
     hip.check(L.scir_hip_tune_set(b"direct.loader", loader))
     hip.check(L.scir_hip_tune_set(b"direct.r", r))
     hip.check(L.scir_hip_tune_set(b"direct.seg_tiles", seg))
+    hip.check(L.scir_hip_tune_set(b"direct.dma_bufs", bufs))
     yield request.param
-    for key in (b"direct.loader", b"direct.r", b"direct.seg_tiles"):
+    for key in (b"direct.loader", b"direct.r", b"direct.seg_tiles", b"direct.dma_bufs"):
         hip.check(L.scir_hip_tune_set(key, -1))
 
 
