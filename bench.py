@@ -320,7 +320,8 @@ def run_vjp(args):
                 wall_s=round(wall, 3),
                 roofline=dict(bound="fp32", achieved=round(ach, 2), peak=FP32_PEAK_TFLOPS,
                               unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 4),
-                              kernel="fir_vjp_db_partial_kernel", traffic=None))
+                              kernel="fir_vjp_db_partial_kernel",
+                              traffic=load_traffic(f"fir_vjp_db_{prec}_vjp_k255_pmc")))
 
 
 def run_copy(args):
@@ -348,7 +349,9 @@ def main():
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps before the timed ones (default: enough for about "
+                         "25 ms of work, past the clock transient after a load step)")
     ap.add_argument("--workload", default=HEADLINE,
                     choices=sorted(DENSE) + ["ragged", "vjp_k255", "copy_1gib"])
     ap.add_argument("--precision", default=None)
@@ -362,6 +365,11 @@ def main():
               "Rows and ragged signals shard across GPUs with no exchange; only the one-GPU "
               "figure is measured here.", file=sys.stderr)
         sys.exit(2)
+    if args.warmup is None:
+        # ~25 ms of untimed work per workload: the shader clock dips and
+        # recovers over the first ~20 ms of a sustained load.
+        args.warmup = {"direct_k33": 60, "direct_k129": 30, "copy_1gib": 60,
+                       "ragged": 2}.get(args.workload, 20)
     import torch
     from scir_amd import _lib
     torch.cuda.set_device(0)
