@@ -25,8 +25,10 @@ after the timed region on the same inputs:
                                       rows of the same workload (~10 s)
 
 Other workloads (``--workload``): longtap_k2048, direct_k33, direct_k129,
-ragged (10000 CSR signals), vjp_k255 (forward + dL/dx + dL/db), copy_1gib
-(plain device copy: attainable bandwidth and HBM-counter calibration).
+ragged (10000 CSR signals), vjp_k255 (forward + dL/dx + dL/db), resample_2_3
+(float64 polyphase resampler, x [4096, 65536], up 2, down 3, the reference's
+31-tap filter), copy_1gib (plain device copy: attainable bandwidth and
+HBM-counter calibration).
 """
 from __future__ import annotations
 
@@ -324,6 +326,62 @@ def run_vjp(args):
                               traffic=load_traffic(f"fir_vjp_db_{prec}_vjp_k255_pmc")))
 
 
+def run_resample(args):
+    """Batched float64 polyphase resampling (up 2, down 3, 31 taps): HBM
+    bytes = 8 per input sample read + 8 per output sample written."""
+    import ctypes
+    import torch
+    from scir_amd import _lib, signal
+    B, N, up, down = 4096, 65536, 2, 3
+    h = signal.RESAMPLE_H_2_3
+    M = signal.resample_len(N, up, down, h.size)
+    g = torch.Generator(device="cuda").manual_seed(0x5C120006)
+    x = torch.randn((B, N), dtype=torch.float64, device="cuda", generator=g)
+    hd = torch.tensor(h, device="cuda")
+    y = torch.empty((B, M), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    L = _lib.lib()
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())
+
+    def step():
+        _lib.check(L.scir_hip_resample_poly_f64_dev(vp(x), N, B, N, up, down, vp(hd), h.size,
+                                                     vp(y), M, ctypes.c_void_p(stream.cuda_stream)))
+    ms, per, wall = timed(step, args.steps, args.warmup, stream)
+    alg = 8.0 * B * (N + M)
+    gbs = alg / (float(np.mean(per)) * 1e-3) / 1e9
+    out = dict(metric="polyphase resample Gsamples/s (output samples)",
+               value=round(B * M / (ms * 1e-3) / 1e9, 2), unit="Gsamples/s", n_gpus=1,
+               steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 4),
+               higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+               data="synthetic",
+               config=dict(workload="resample_2_3", rows=B, samples_per_row=N, up=up, down=down,
+                           taps=int(h.size), outputs_per_row=M),
+               wall_s=round(wall, 3),
+               roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                             frac=round(gbs / HBM_PEAK_GBS, 4),
+                             traffic=load_traffic("resample_poly_f64_resample_2_3_pmc")))
+    rows = [0, B - 1]
+    from oracle import oracle as orc
+    xs = x[rows].cpu().numpy()
+    err = max(float(np.max(np.abs(y[r].cpu().numpy() - orc.resample_poly_f64(xs[i], up, down, h))))
+              for i, r in enumerate(rows))
+    out["max_abs_err_vs_reference_order"] = err
+    if not args.no_cpu_baseline:
+        # the oracle's restatement of the reference loop, one thread, whole rows
+        xh = x[:64].cpu().numpy()
+        t0 = time.perf_counter()
+        done = 0
+        while time.perf_counter() - t0 < args.cpu_budget:  # cycles over the 64 rows
+            orc.resample_poly_f64(xh[done % 64], up, down, h)
+            done += 1
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(value=round(done * M / dt / 1e9, 6), unit="Gsamples/s",
+                                   cores=1, kind="port",
+                                   sample=f"{done} whole rows (64 distinct) of the workload, "
+                                          f"1 thread, {dt:.1f} s")
+    return out
+
+
 def run_copy(args):
     """Plain device copy of 1 GiB (2^28 floats): the attainable-bandwidth
     reference and the calibration kernel for the HBM byte counters."""
@@ -353,7 +411,7 @@ def main():
                     help="untimed steps before the timed ones (default: enough for about "
                          "25 ms of work, past the clock transient after a load step)")
     ap.add_argument("--workload", default=HEADLINE,
-                    choices=sorted(DENSE) + ["ragged", "vjp_k255", "copy_1gib"])
+                    choices=sorted(DENSE) + ["ragged", "vjp_k255", "resample_2_3", "copy_1gib"])
     ap.add_argument("--precision", default=None)
     ap.add_argument("--quick", action="store_true",
                     help="skip the side comparisons (and, for ragged, the per-signal launches)")
@@ -380,6 +438,8 @@ def main():
         out = run_vjp(args)
     elif args.workload == "copy_1gib":
         out = run_copy(args)
+    elif args.workload == "resample_2_3":
+        out = run_resample(args)
     else:
         out = run_dense(args)
     print(json.dumps(out), flush=True)

@@ -115,6 +115,38 @@ SCIR_HIP_API int scir_hip_fir_f32_cpu(const float *x, int64_t ldx, const float *
                          int64_t B, int64_t N, float *y, int64_t ldy,
                          int32_t nthreads);
 
+/* ---- polyphase resampling (float64) -----------------------------------------
+ * Replaces the reference's CPU-only resample_poly
+ * (crates/scir-signal/src/lib.rs:313-362: x upsampled by `up` with zeros,
+ * fully convolved with a K-tap filter h, every `down`-th sample kept from
+ * off = (K-1)/2 while below len(conv) - off), generalised to batches of rows
+ * and to any up, down >= 1 and h (the reference fixes up = 2, down = 3 and a
+ * 31-tap table).  Row r of y holds M outputs,
+ *   y[r, m] = sum_{k = n mod up, n = off + down*m; k < K; k += up} h[k] * x[r, (n-k)/up]
+ * with x[r, s] = 0 outside [0, N).  M is given by scir_hip_resample_len. */
+SCIR_HIP_API int scir_hip_resample_len(int64_t N, int32_t up, int32_t down, int32_t K,
+                                       int64_t *M);
+/* Device tier: x [B, ldx >= N], y [B, ldy >= M] float64 device pointers,
+ * h [K] float64 device pointer (K <= 8192), enqueued on `stream`. */
+SCIR_HIP_API int scir_hip_resample_poly_f64_dev(const double *x, int64_t ldx, int64_t B,
+                                                int64_t N, int32_t up, int32_t down,
+                                                const double *h, int32_t K, double *y,
+                                                int64_t ldy, void *stream);
+/* Host tier: host arrays in and out (library-owned device buffers, copies
+ * and launch on the library's stream, synchronous), like scir_hip_fir_f32_host. */
+SCIR_HIP_API int scir_hip_resample_poly_f64_host(const double *x, int64_t ldx, int64_t B,
+                                                 int64_t N, int32_t up, int32_t down,
+                                                 const double *h, int32_t K, double *y,
+                                                 int64_t ldy);
+/* CPU arm: the reference's operation order (each output accumulates h[k] *
+ * x terms in ascending x index, multiply and add rounded separately), so
+ * for the reference's filter table the result is bit-identical to
+ * resample_poly; rows spread over `nthreads` host threads (<= 0: all). */
+SCIR_HIP_API int scir_hip_resample_poly_f64_cpu(const double *x, int64_t ldx, int64_t B,
+                                                int64_t N, int32_t up, int32_t down,
+                                                const double *h, int32_t K, double *y,
+                                                int64_t ldy, int32_t nthreads);
+
 /* ---- ragged (CSR) batches -------------------------------------------------
  * No counterpart in the reference (its FIR takes one dense Array2).  Signal s
  * is x[offs[s] .. offs[s+1]) with taps b[b_offs[s] .. b_offs[s+1]); y has the

@@ -26,6 +26,10 @@
  *   oracle_fir_ref_f64   scipy lfilter golden vectors (1e-12), brute force.
  *   oracle_lfilter_f64   scipy lfilter golden vectors, == ref_f64 on reversed taps.
  *   oracle_fir_absbound  closed form on |x|, |b| (a sum of absolute values).
+ *   oracle_resample_poly_f64  scipy resample_poly golden output for the
+ *                        reference's fixture input within the reference test's
+ *                        own tolerance, brute-force polyphase sums, impulse
+ *                        responses, the reference's output length.
  *   oracle_fir_vjp_f64   central finite differences of sum(w*y) (exact for a
  *                        bilinear loss up to rounding) and torch float64
  *                        autograd through conv1d.
@@ -152,4 +156,39 @@ void oracle_lfilter_ragged_f64(const float *x, const int64_t *offs, const float 
         int64_t k0 = boffs[s], K = boffs[s + 1] - boffs[s];
         oracle_lfilter_f64(x + n0, len, b + k0, K, 1, len, y + n0, len);
     }
+}
+
+/* crates/scir-signal/src/lib.rs:293-302 (convolve) and :313-362
+ * (resample_poly), generalised from the fixed up = 2, down = 3 and 31-tap H
+ * to any up, down >= 1 and taps h[K], K >= 1, with the same steps and
+ * operation order: x is upsampled by inserting up-1 zeros after every
+ * sample; the full linear convolution with h is accumulated as
+ * conv[i + j] += xu[i] * h[j] for i ascending, then j ascending (separately
+ * rounded multiply and add); outputs are conv[off + down * m] for
+ * off = (K - 1) / 2 while off + down * m < len(conv) - off.
+ * Returns the number of outputs written to y (y must hold
+ * oracle_resample_len(n, up, down, K) doubles); `work` must hold
+ * n * up + K - 1 doubles. */
+int64_t oracle_resample_len(int64_t n, int64_t up, int64_t down, int64_t K)
+{
+    const int64_t len = n * up + K - 1, off = (K - 1) / 2, end = len - off;
+    if (end <= off) return 0;
+    return (end - off + down - 1) / down;
+}
+
+int64_t oracle_resample_poly_f64(const double *x, int64_t n, int64_t up, int64_t down,
+                                 const double *h, int64_t K, double *work, double *y)
+{
+    const int64_t nu = n * up, len = nu + K - 1, off = (K - 1) / 2, end = len - off;
+    for (int64_t i = 0; i < len; ++i) work[i] = 0.0;
+    for (int64_t i = 0; i < nu; ++i) {
+        const double xi = (i % up == 0) ? x[i / up] : 0.0;
+        for (int64_t j = 0; j < K; ++j) {
+            const double p = xi * h[j];
+            work[i + j] = work[i + j] + p;
+        }
+    }
+    int64_t m = 0;
+    for (int64_t idx = off; idx < end; idx += down) y[m++] = work[idx];
+    return m;
 }

@@ -14,6 +14,8 @@ Functions and the reference lines they restate:
 * :func:`abs_bound`    -- ``sum_k |b_k| |x_{n-k}|`` for the error bound.
 * :func:`vjp_f64`      -- FP64 reference gradient of ``sum(g * y)``.
 * :func:`lfilter_ragged_f64` -- the CSR batch form of :func:`lfilter_f64`.
+* :func:`resample_poly_f64` -- crates/scir-signal/src/lib.rs:293-362
+  (``convolve`` + ``resample_poly``), for any up/down and taps.
 
 Every function is pinned by tests/test_oracle.py (see the C file header).
 """
@@ -60,6 +62,11 @@ def _load():
                    lib.oracle_fir_absbound, lib.oracle_fir_vjp_f64,
                    lib.oracle_lfilter_ragged_f64):
             fn.restype = None
+        lib.oracle_resample_len.argtypes = [_i64, _i64, _i64, _i64]
+        lib.oracle_resample_len.restype = _i64
+        lib.oracle_resample_poly_f64.argtypes = [_f64p, _i64, _i64, _i64, _f64p, _i64, _f64p,
+                                                 _f64p]
+        lib.oracle_resample_poly_f64.restype = _i64
         _lib = lib
     return _lib
 
@@ -146,6 +153,25 @@ def lfilter_ragged_f64(x, offs, b, boffs) -> np.ndarray:
     y = np.zeros(x.size, np.float64)
     _load().oracle_lfilter_ragged_f64(x, offs, b, boffs, offs.size - 1, y)
     return y
+
+
+def resample_len(n: int, up: int, down: int, K: int) -> int:
+    """Number of outputs of :func:`resample_poly_f64` for n input samples."""
+    return int(_load().oracle_resample_len(n, up, down, K))
+
+
+def resample_poly_f64(x, up: int, down: int, h) -> np.ndarray:
+    """Polyphase resampling of one float64 signal, restating the reference's
+    upsample / full convolution / decimate steps in their operation order
+    (bitwise equal to that loop).  ``h`` is the anti-aliasing filter (the
+    reference uses 31 taps, up=2, down=3)."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    h = np.ascontiguousarray(h, dtype=np.float64).ravel()
+    n, K = x.size, h.size
+    work = np.zeros(max(1, n * up + K - 1), np.float64)
+    y = np.zeros(max(1, resample_len(n, up, down, K)), np.float64)
+    m = _load().oracle_resample_poly_f64(x, n, up, down, h, K, work, y)
+    return y[:m]
 
 
 def rel_l2(y, y64) -> float:

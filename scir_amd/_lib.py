@@ -64,6 +64,13 @@ SIGNATURES = {
     "scir_hip_autotune_load": (_I, [_c.c_char_p]),
     "scir_hip_tune_set": (_I, [_c.c_char_p, _I32]),
     "scir_hip_synth_normal_f32": (_I, [_P, _I64, _I64, _I64, _U64, _F, _F, _P]),
+    "scir_hip_resample_len": (_I, [_I64, _I32, _I32, _I32, _Pi64]),
+    "scir_hip_resample_poly_f64_dev": (_I, [_P, _I64, _I64, _I64, _I32, _I32, _P, _I32, _P,
+                                            _I64, _P]),
+    "scir_hip_resample_poly_f64_cpu": (_I, [_P, _I64, _I64, _I64, _I32, _I32, _P, _I32, _P,
+                                            _I64, _I32]),
+    "scir_hip_resample_poly_f64_host": (_I, [_P, _I64, _I64, _I64, _I32, _I32, _P, _I32, _P,
+                                             _I64]),
 }
 
 

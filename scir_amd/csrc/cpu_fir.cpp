@@ -82,3 +82,66 @@ extern "C" int scir_hip_fir_f32_cpu(const float *x, int64_t ldx, const float *b,
     for (auto &t : pool) t.join();
     return SCIR_HIP_OK;
 }
+
+// CPU arm of the polyphase resampler, in the reference's operation order
+// (crates/scir-signal/src/lib.rs:293-362): output m is entry n = off + down*m
+// of the full convolution of the zero-stuffed signal with h, which the
+// reference accumulates as acc = acc + x[s] * h[n - up*s] over s ascending
+// (its inner loop also adds the products of the inserted zeros; adding +-0
+// to a sum that is never -0.0 leaves every bit unchanged, so they are
+// skipped here).
+namespace {
+
+void resample_row(const double *xr, int64_t N, int64_t up, int64_t down, const double *h,
+                  int64_t K, double *yr, int64_t M)
+{
+    const int64_t off = (K - 1) / 2;
+    for (int64_t m = 0; m < M; ++m) {
+        const int64_t n = off + down * m;
+        // s from ceil((n - K + 1) / up) (but >= 0) to min(n / up, N - 1)
+        const int64_t lo_num = n - K + 1;
+        int64_t s_lo = lo_num <= 0 ? 0 : (lo_num + up - 1) / up;
+        const int64_t s_hi = std::min<int64_t>(n / up, N - 1);
+        double acc = 0.0;
+        for (int64_t s = s_lo; s <= s_hi; ++s) {
+            const double p = xr[s] * h[n - up * s];
+            acc = acc + p;
+        }
+        yr[m] = acc;
+    }
+}
+
+}  // namespace
+
+extern "C" int scir_hip_resample_len(int64_t N, int32_t up, int32_t down, int32_t K, int64_t *M);
+
+extern "C" int scir_hip_resample_poly_f64_cpu(const double *x, int64_t ldx, int64_t B, int64_t N,
+                                              int32_t up, int32_t down, const double *h,
+                                              int32_t K, double *y, int64_t ldy,
+                                              int32_t nthreads)
+{
+    int64_t M = 0;
+    const int st = scir_hip_resample_len(N, up, down, K, &M);
+    if (st != SCIR_HIP_OK) return st;
+    if (B < 0) return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "negative row count");
+    if (B == 0 || M == 0) return SCIR_HIP_OK;
+    if (!x || !y || !h)
+        return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "null pointer argument");
+    if (ldx < N || ldy < M)
+        return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row stride smaller than the row length");
+    int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
+    nt = (int)std::min<int64_t>(nt, B);
+    auto work = [&](int w) {
+        for (int64_t r = w; r < B; r += nt)
+            resample_row(x + r * ldx, N, up, down, h, K, y + r * ldy, M);
+    };
+    if (nt <= 1) {
+        work(0);
+        return SCIR_HIP_OK;
+    }
+    std::vector<std::thread> pool;
+    pool.reserve(nt);
+    for (int w = 0; w < nt; ++w) pool.emplace_back(work, w);
+    for (auto &t : pool) t.join();
+    return SCIR_HIP_OK;
+}

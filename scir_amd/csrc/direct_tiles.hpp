@@ -8,8 +8,14 @@
 // copy, applying taps 32 at a time from scalar registers; remainders use
 // chunks of 16, 8, 4, 2, 1 taps so no tap is padded.
 //
-// LDS layout: sample s lives at s + 4*(s/16) (4 pad words after every 16),
-// which makes the 16-float lane stride of the window reads bank-conflict free.
+// LDS layout: linear, sample s of the staged window at float s.  A window
+// read (16 lanes 32 bytes apart, 16-byte loads) meets 2-way bank conflicts,
+// but every read address is the chunk's base plus a constant offset, which
+// saves the VALU instructions a padded or swizzled address costs: 257
+// instead of 296 VALU instructions per 32 taps and 8 outputs, measured
+// 384 vs ~395 us at K = 33 and 97.7 % of the non-packed FP32 peak at
+// K >= 1000.  Tiles staged by LDS DMA (64 consecutive 16-byte chunks per
+// wave instruction) cannot be padded at all.
 //
 // EXACT = true keeps the reference's arithmetic: separately rounded multiply
 // and add, taps accumulated in the order k = 0..K-1 from +0.0f
@@ -30,22 +36,6 @@ constexpr int kChunk = 32;    // taps per fully unrolled step
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ int pad16(int s) { return s + ((s >> 4) << 2); }
-
-// Second LDS layout, for tiles written by LDS DMA (which fills 64 consecutive
-// 16-byte chunks per wave instruction, so no padding can be inserted): sample
-// s at s.  The window reads (16 lanes 32 bytes apart) then meet 2-way bank
-// conflicts, but every read address is the chunk's base plus a constant, which
-// saves the VALU instructions a padded or swizzled address costs (measured:
-// 257 instead of 296 VALU instructions per 32 taps and 8 outputs).
-enum Layout { kPad16 = 0, kLinear = 1 };
-
-template <int LAYOUT>
-__device__ __forceinline__ int lds_index(int s)
-{
-    return LAYOUT == kLinear ? s : pad16(s);
-}
-
 template <bool EXACT>
 __device__ __forceinline__ float mac(float acc, float t, float xv);
 
@@ -64,9 +54,9 @@ __device__ __forceinline__ float mac<false>(float acc, float t, float xv)
 }
 
 // acc[r] += sum_{kk < C} bk[kk] * x[i_r - k0 - kk], where the LDS window
-// starts OFF floats below the 4-aligned unpadded index `wa`:
+// starts OFF floats below the 4-aligned index `wa`:
 //   x[i_r - k0 - kk] == xs[wa + OFF + r + C - kk].
-template <int R, int C, int OFF, bool EXACT, int LAYOUT = kPad16>
+template <int R, int C, int OFF, bool EXACT>
 __device__ __forceinline__ void taps_chunk(float (&acc)[R], const float *xs, int wa,
                                            const float *__restrict__ bk)
 {
@@ -74,7 +64,7 @@ __device__ __forceinline__ void taps_chunk(float (&acc)[R], const float *xs, int
     float w[W4 * 4];
 #pragma unroll
     for (int j = 0; j < W4; ++j) {
-        f32x4 v = *reinterpret_cast<const f32x4 *>(xs + lds_index<LAYOUT>(wa + 4 * j));
+        f32x4 v = *reinterpret_cast<const f32x4 *>(xs + wa + 4 * j);
         w[4 * j + 0] = v.x;
         w[4 * j + 1] = v.y;
         w[4 * j + 2] = v.z;
@@ -119,42 +109,42 @@ __device__ __forceinline__ void stage_row(float *xs, const float *__restrict__ x
                                           int count, int64_t N, bool vec_ok, bool rev)
 {
     for (int v = threadIdx.x; v < count / 4; v += kThreads)
-        *reinterpret_cast<f32x4 *>(xs + pad16(4 * v)) =
+        *reinterpret_cast<f32x4 *>(xs + 4 * v) =
             load4(xrow, g0 + 4 * (int64_t)v, N, vec_ok, rev);
 }
 
 // acc[r] += sum_{k' < kcount} bs[k'] * x[i_r - k'] with the window in LDS:
 // x[i_r - k'] == xs[H + lane_base + r - k'].  Chunks run in ascending k'.
-template <int R, bool EXACT, int LAYOUT = kPad16>
+template <int R, bool EXACT>
 __device__ __forceinline__ void taps_from_lds(float (&acc)[R], const float *xs, int H,
                                               int lane_base, int kcount,
                                               const float *__restrict__ bs)
 {
     int kc = 0;
     for (; kc + kChunk <= kcount; kc += kChunk)
-        taps_chunk<R, kChunk, 0, EXACT, LAYOUT>(acc, xs, H + lane_base - kc - kChunk, bs + kc);
+        taps_chunk<R, kChunk, 0, EXACT>(acc, xs, H + lane_base - kc - kChunk, bs + kc);
     const int rem = kcount - kc;
     if (rem & 16) {
-        taps_chunk<R, 16, 0, EXACT, LAYOUT>(acc, xs, H + lane_base - kc - 16, bs + kc);
+        taps_chunk<R, 16, 0, EXACT>(acc, xs, H + lane_base - kc - 16, bs + kc);
         kc += 16;
     }
     if (rem & 8) {
-        taps_chunk<R, 8, 0, EXACT, LAYOUT>(acc, xs, H + lane_base - kc - 8, bs + kc);
+        taps_chunk<R, 8, 0, EXACT>(acc, xs, H + lane_base - kc - 8, bs + kc);
         kc += 8;
     }
     if (rem & 4) {
-        taps_chunk<R, 4, 0, EXACT, LAYOUT>(acc, xs, H + lane_base - kc - 4, bs + kc);
+        taps_chunk<R, 4, 0, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
         kc += 4;
     }
     if (rem & 2) {  // kc % 4 == 0 here; window starts 2 below a 4-aligned index
-        taps_chunk<R, 2, 2, EXACT, LAYOUT>(acc, xs, H + lane_base - kc - 4, bs + kc);
+        taps_chunk<R, 2, 2, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
         kc += 2;
     }
     if (rem & 1) {
         if (kc & 2)  // kc % 4 == 2: window starts 1 below the aligned index
-            taps_chunk<R, 1, 1, EXACT, LAYOUT>(acc, xs, H + lane_base - kc - 2, bs + kc);
+            taps_chunk<R, 1, 1, EXACT>(acc, xs, H + lane_base - kc - 2, bs + kc);
         else         // kc % 4 == 0: window starts 3 below the aligned index
-            taps_chunk<R, 1, 3, EXACT, LAYOUT>(acc, xs, H + lane_base - kc - 4, bs + kc);
+            taps_chunk<R, 1, 3, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
     }
 }
 

@@ -18,8 +18,7 @@
 //   * long filters are processed in super-chunks of 2048 taps so the LDS
 //     footprint stays bounded for any K.
 //
-// LDS layout: sample s lives at s + 4*(s/16) (4 pad words after every 16),
-// which makes the 16-float lane stride of the window reads bank-conflict free.
+// LDS layout: linear (see direct_tiles.hpp).
 //
 // Two arithmetic modes (template EXACT):
 //   EXACT = false: fused multiply-add in any order -- error within
@@ -110,7 +109,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_pipe_kernel(
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             const int v = threadIdx.x + s * kThreads;
-            if (v < count4) *reinterpret_cast<f32x4 *>(buf + pad16(4 * v)) = pre[s];
+            if (v < count4) *reinterpret_cast<f32x4 *>(buf + 4 * v) = pre[s];
         }
         __syncthreads();
         if (it + 1 < ntiles) fetch(tfirst + it + 1);
@@ -134,7 +133,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_pipe_kernel(
 //
 // Wave w copies the LDS chunk groups q = w, w+4, ... (64 chunks of 16 bytes,
 // one instruction each) with 64*q < count4; lane l of group q fills tile
-// chunk 64q + l (the linear LDS layout of direct_tiles.hpp).
+// chunk 64q + l (the linear LDS layout).
 __device__ __forceinline__ void dma_chunk16(u32x4 rsrc, unsigned voffset, unsigned lds_byte)
 {
     // M0 (the LDS destination of the wave) is written by the compiler just
@@ -223,7 +222,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_dma_kernel(
         float acc[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) acc[r] = 0.0f;
-        taps_from_lds<R, EXACT, kLinear>(acc, buf, H, lane_base, K, b);
+        taps_from_lds<R, EXACT>(acc, buf, H, lane_base, K, b);
         const int64_t i0 = (tfirst + it) * T + lane_base;
 #pragma unroll
         for (int r = 0; r < R; r += 4)
@@ -426,13 +425,21 @@ int scir_hip_fir_f32_host(const float *x, int64_t ldx, const float *b, int32_t K
     float *dx = static_cast<float *>(st->host_x.ptr);
     float *dy = static_cast<float *>(st->host_y.ptr);
     float *db = static_cast<float *>(st->host_b.ptr);
-    SCIR_HIP_TRY(hipMemcpy2DAsync(dx, ldd * sizeof(float), x, ldx * sizeof(float), row_bytes, B,
-                                  hipMemcpyHostToDevice, s));
+    // One linear copy when host and device rows are both packed (a pitched
+    // copy of many rows runs as many small transfers).
+    if (ldx == N && ldd == N)
+        SCIR_HIP_TRY(hipMemcpyAsync(dx, x, mat_bytes, hipMemcpyHostToDevice, s));
+    else
+        SCIR_HIP_TRY(hipMemcpy2DAsync(dx, ldd * sizeof(float), x, ldx * sizeof(float), row_bytes,
+                                      B, hipMemcpyHostToDevice, s));
     if (K > 0)
         SCIR_HIP_TRY(hipMemcpyAsync(db, b, (size_t)K * sizeof(float), hipMemcpyHostToDevice, s));
     SCIR_TRY(fir_dense_dispatch(dx, ldd, db, K, B, N, dy, ldd, precision, false, s, st));
-    SCIR_HIP_TRY(hipMemcpy2DAsync(y, ldy * sizeof(float), dy, ldd * sizeof(float), row_bytes, B,
-                                  hipMemcpyDeviceToHost, s));
+    if (ldy == N && ldd == N)
+        SCIR_HIP_TRY(hipMemcpyAsync(y, dy, mat_bytes, hipMemcpyDeviceToHost, s));
+    else
+        SCIR_HIP_TRY(hipMemcpy2DAsync(y, ldy * sizeof(float), dy, ldd * sizeof(float), row_bytes,
+                                      B, hipMemcpyDeviceToHost, s));
     SCIR_HIP_TRY(hipStreamSynchronize(s));
     return SCIR_HIP_OK;
 }

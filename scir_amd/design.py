@@ -102,3 +102,14 @@ def lowpass(numtaps: int, cutoff: float = 0.2) -> np.ndarray:
 def bandpass_kaiser(numtaps: int, band=(0.1, 0.3), beta: float = 8.0) -> np.ndarray:
     """Kaiser(beta)-windowed band-pass, unit gain at the band centre, float32."""
     return firwin(numtaps, list(band), ("kaiser", beta), pass_zero=False).astype(np.float32)
+
+
+def resample_filter(up: int, down: int, numtaps: int = 31) -> np.ndarray:
+    """Anti-aliasing filter of the polyphase resampler (float64): a Hamming
+    low-pass at 1/max(up, down) of Nyquist with gain ``up``.  For up=2,
+    down=3 and 31 taps this is the fixed table of the reference's
+    resample_poly (crates/scir-signal/src/lib.rs:315-346) to within one ulp."""
+    q = max(int(up), int(down))
+    if q < 2:
+        raise ValueError("resampling needs up or down >= 2")
+    return float(up) * firwin(numtaps, 1.0 / q)

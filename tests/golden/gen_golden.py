@@ -12,6 +12,11 @@ Files (all float64 unless noted, saved with allow_pickle=False):
   lfilter_x.npy            float32 inputs [B, N]
   lfilter_b<K>.npy         float32 lfilter-order taps used with lfilter_x
   lfilter_y<K>.npy         scipy.signal.lfilter(b, [1.0], x) in float64
+  resample_x.npy           np.linspace(0, 1, 32, endpoint=False), the input of the
+                           reference's fixture script (scripts/gen_signal_fixtures.py:28)
+  resample_poly_2_3.npy    scipy.signal.resample_poly(resample_x, 2, 3), the expected
+                           output of the reference's resample_poly test
+                           (crates/scir-signal/src/lib.rs:655-668, atol 2e-2)
 """
 import pathlib
 
@@ -29,6 +34,7 @@ FIRWIN = {
     "lp15_c005": dict(numtaps=15, cutoff=0.05),
     "lp1023_c045": dict(numtaps=1023, cutoff=0.45),
     "lp64_c0317": dict(numtaps=64, cutoff=0.317),
+    "lp31_c0333": dict(numtaps=31, cutoff=1.0 / 3.0),
 }
 
 
@@ -45,6 +51,10 @@ def main() -> None:
         y = signal.lfilter(b.astype(np.float64), [1.0], x.astype(np.float64), axis=-1)
         np.save(OUT / f"lfilter_b{K}.npy", b, allow_pickle=False)
         np.save(OUT / f"lfilter_y{K}.npy", y, allow_pickle=False)
+
+    xr = np.linspace(0, 1, 32, endpoint=False)
+    np.save(OUT / "resample_x.npy", xr, allow_pickle=False)
+    np.save(OUT / "resample_poly_2_3.npy", signal.resample_poly(xr, 2, 3), allow_pickle=False)
 
 
 if __name__ == "__main__":
