@@ -26,38 +26,86 @@ constexpr int kJ = 4;                     // outputs per lane
 constexpr int kT = kThreads * kJ;         // outputs per workgroup
 constexpr int kMaxTaps = 8192;
 
+// UNI: every output of a lane has the same phase (up divides 256*down): the
+// lane reads each tap once for its kJ outputs.
+template <bool UNI>
 __global__ __launch_bounds__(kThreads) void resample_poly_kernel(
     const double *__restrict__ x, int64_t ldx, int64_t N, double *__restrict__ y, int64_t ldy,
     int64_t M, const double *__restrict__ h, int K, int up, int down, int tiles_per_row,
     int span)
 {
     extern __shared__ __attribute__((aligned(16))) double lds_d[];
-    double *hs = lds_d;           // K taps
-    double *xs = lds_d + K;       // span samples: x[s0 .. s0 + span)
+    double *hs = lds_d;                      // K taps
+    double *xs = lds_d + ((K + 1) & ~1);     // span samples x[s0 ..], 16-byte aligned
     const int64_t row = blockIdx.x / tiles_per_row;
     const int64_t m0 = (int64_t)(blockIdx.x % tiles_per_row) * kT;
     const int64_t off = (K - 1) / 2;
     const int tmax = (K + up - 1) / up;   // taps of the longest phase
-    // First sample the tile reads: q(m0) - (tmax - 1).
-    const int64_t s0 = (off + down * m0) / up - (tmax - 1);
+    const int64_t n0 = off + down * m0;   // convolution index of the tile's first output
+    // First sample the tile reads: n0 / up - (tmax - 1).
+    const int64_t s0 = n0 / up - (tmax - 1);
+    const int r0 = (int)(n0 % up);        // phase of output m0
     const double *__restrict__ xr = x + row * ldx;
     for (int k = threadIdx.x; k < K; k += kThreads) hs[k] = h[k];
-    for (int i = threadIdx.x; i < span; i += kThreads) {
-        const int64_t s = s0 + i;
-        xs[i] = (s >= 0 && s < N) ? xr[s] : 0.0;
+    // Stage x[s0 .. s0 + span): pairs of samples where both are inside the
+    // row and 16-byte aligned, single samples otherwise.
+    const bool pairs = ((reinterpret_cast<uintptr_t>(xr) + 8 * s0) & 15) == 0;
+    if (pairs) {
+        for (int i = 2 * threadIdx.x; i < span; i += 2 * kThreads) {
+            const int64_t s = s0 + i;
+            if (s >= 0 && s + 1 < N) {
+                const double2 v = *reinterpret_cast<const double2 *>(xr + s);
+                xs[i] = v.x;
+                if (i + 1 < span) xs[i + 1] = v.y;
+            } else {
+                xs[i] = (s >= 0 && s < N) ? xr[s] : 0.0;
+                if (i + 1 < span) xs[i + 1] = (s + 1 >= 0 && s + 1 < N) ? xr[s + 1] : 0.0;
+            }
+        }
+    } else {
+        for (int i = threadIdx.x; i < span; i += kThreads) {
+            const int64_t s = s0 + i;
+            xs[i] = (s >= 0 && s < N) ? xr[s] : 0.0;
+        }
     }
     __syncthreads();
     double *__restrict__ yr = y + row * ldy;
+    // Output m0 + d: convolution index n0 + down*d = up*(s0 + tmax - 1) + r0 + down*d,
+    // so its phase is (r0 + down*d) % up and x[q] sits at xs[tmax - 1 + (r0 + down*d) / up].
+    if (UNI) {
+        const int d0 = threadIdx.x;
+        const int ph = (r0 + down * d0) % up;
+        int qi[kJ];
+        bool live[kJ];
 #pragma unroll
-    for (int j = 0; j < kJ; ++j) {
-        const int64_t m = m0 + threadIdx.x + kThreads * j;
-        if (m >= M) break;
-        const int64_t n = off + down * m;
-        const int ph = (int)(n % up);
-        const int qi = (int)(n / up - s0);  // index of x[q] in xs
-        double acc = 0.0;
-        for (int k = ph, t = 0; k < K; k += up, ++t) acc = __builtin_fma(hs[k], xs[qi - t], acc);
-        yr[m] = acc;
+        for (int j = 0; j < kJ; ++j) {
+            const int d = d0 + kThreads * j;
+            qi[j] = tmax - 1 + (r0 + down * d) / up;
+            live[j] = m0 + d < M;
+        }
+        double acc[kJ];
+#pragma unroll
+        for (int j = 0; j < kJ; ++j) acc[j] = 0.0;
+        for (int k = ph, t = 0; k < K; k += up, ++t) {
+            const double hk = hs[k];
+#pragma unroll
+            for (int j = 0; j < kJ; ++j) acc[j] = __builtin_fma(hk, xs[qi[j] - t], acc[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kJ; ++j)
+            if (live[j]) yr[m0 + d0 + kThreads * j] = acc[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < kJ; ++j) {
+            const int d = threadIdx.x + kThreads * j;
+            if (m0 + d >= M) break;
+            const int nl = r0 + down * d;
+            const int ph = nl % up;
+            const int qi = tmax - 1 + nl / up;
+            double acc = 0.0;
+            for (int k = ph, t = 0; k < K; k += up, ++t) acc = __builtin_fma(hs[k], xs[qi - t], acc);
+            yr[m0 + d] = acc;
+        }
     }
 }
 
@@ -90,24 +138,33 @@ int resample_launch(const double *x, int64_t ldx, int64_t B, int64_t N, int32_t 
     // Samples a tile reads: q(m0 + T - 1) - q(m0) + tmax <= down*(T-1)/up + 1 + tmax.
     const int tmax = (K + up - 1) / up;
     const int span = (int)(((int64_t)down * (kT - 1)) / up + 2 + tmax);
-    const size_t lds = ((size_t)K + span) * sizeof(double);
+    const size_t lds = ((size_t)((K + 1) & ~1) + span + 1) * sizeof(double);
     if (lds > 160 * 1024)
         return fail(SCIR_HIP_ERR_UNSUPPORTED, "K=%d, up=%d, down=%d need too much LDS", K, up,
                     down);
     static bool attr_set = false;  // guarded by the device mutex
     if (!attr_set) {
-        SCIR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&resample_poly_kernel),
+        SCIR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&resample_poly_kernel<true>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        SCIR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&resample_poly_kernel<false>),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr_set = true;
     }
+    // Every output of a lane has one phase when up divides kThreads * down.
+    const bool uni = ((int64_t)kThreads * down) % up == 0;
     const int64_t tiles = (M + kT - 1) / kT;
     if (tiles > 0x7fffffff) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row too long");
     const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / tiles);
     for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
         const int64_t rows = std::min(rows_per_launch, B - r0);
-        hipLaunchKernelGGL(resample_poly_kernel, dim3((unsigned)(rows * tiles)), dim3(kThreads),
-                           lds, s, x + r0 * ldx, ldx, N, y + r0 * ldy, ldy, M, h, K, up, down,
-                           (int)tiles, span);
+        if (uni)
+            hipLaunchKernelGGL(resample_poly_kernel<true>, dim3((unsigned)(rows * tiles)),
+                               dim3(kThreads), lds, s, x + r0 * ldx, ldx, N, y + r0 * ldy, ldy, M,
+                               h, K, up, down, (int)tiles, span);
+        else
+            hipLaunchKernelGGL(resample_poly_kernel<false>, dim3((unsigned)(rows * tiles)),
+                               dim3(kThreads), lds, s, x + r0 * ldx, ldx, N, y + r0 * ldy, ldy, M,
+                               h, K, up, down, (int)tiles, span);
         SCIR_TRY(check_launch("resample_poly_kernel"));
     }
     return SCIR_HIP_OK;
