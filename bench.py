@@ -27,7 +27,10 @@ after the timed region on the same inputs:
 Other workloads (``--workload``): longtap_k2048, direct_k33, direct_k129,
 ragged (10000 CSR signals), vjp_k255 (forward + dL/dx + dL/db), resample_2_3
 (float64 polyphase resampler, x [4096, 65536], up 2, down 3, the reference's
-31-tap filter), copy_1gib (plain device copy: attainable bandwidth and
+31-tap filter), sosfilt_butter6 (float64 IIR cascade, x [4096, 65536], the
+reference's sosfilt benchmark filter butter(6, 0.15): 3 sections; filtfilt with
+--precision filtfilt, the reference loop's exact order with --precision
+reference_order), copy_1gib (plain device copy: attainable bandwidth and
 HBM-counter calibration).
 """
 from __future__ import annotations
@@ -382,6 +385,76 @@ def run_resample(args):
     return out
 
 
+def run_sosfilt(args):
+    """Batched float64 IIR cascade (the reference's sosfilt benchmark filter,
+    crates/scir-signal/benches/signal.rs:8-18, butter(6, 0.15) = 3 sections) on
+    [4096, 65536].  The call is three launches (chunk states, chunk scan,
+    filtering pass); its algorithmic HBM bytes are 8 per input sample read and
+    8 per output written (16 per sample; the chunked method reads x twice,
+    24 bytes a sample at the counters)."""
+    import ctypes
+    import torch
+    from scipy import signal as ss
+    from scir_amd import _lib
+    B, N = 4096, 65536
+    mode = args.precision or "chunked"
+    if mode not in ("chunked", "reference_order", "filtfilt"):
+        raise SystemExit(f"--precision for sosfilt_butter6: chunked, reference_order or "
+                         f"filtfilt, not {mode}")
+    sos_np = ss.butter(6, 0.15, output="sos")
+    g = torch.Generator(device="cuda").manual_seed(0x5C120007)
+    x = torch.randn((B, N), dtype=torch.float64, device="cuda", generator=g)
+    sos = torch.tensor(sos_np, device="cuda")
+    y = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    L = _lib.lib()
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())
+    fn = L.scir_hip_filtfilt_f64_dev if mode == "filtfilt" else L.scir_hip_sosfilt_f64_dev
+    flags = 1 if mode == "reference_order" else 0
+
+    def step():
+        _lib.check(fn(vp(x), N, B, N, vp(sos), sos_np.shape[0], vp(y), N, flags,
+                      ctypes.c_void_p(stream.cuda_stream)))
+    ms, per, wall = timed(step, args.steps, args.warmup, stream)
+    passes = 2 if mode == "filtfilt" else 1
+    alg = 16.0 * B * N * passes
+    gbs = alg / (float(np.mean(per)) * 1e-3) / 1e9
+    tag = {"chunked": "sosfilt", "reference_order": "sosfilt_refo", "filtfilt": "filtfilt"}[mode]
+    out = dict(metric="sosfilt Gsamples/s", value=round(B * N / (ms * 1e-3) / 1e9, 2),
+               unit="Gsamples/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+               ms_per_step=round(ms, 4), higher_is_better=True, scaling="weak",
+               vs_baseline=None, dtype="f64", data="synthetic",
+               config=dict(workload="sosfilt_butter6", rows=B, samples_per_row=N,
+                           sections=int(sos_np.shape[0]), filter="butter(6, 0.15)", mode=mode),
+               wall_s=round(wall, 3),
+               roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                             frac=round(gbs / HBM_PEAK_GBS, 4),
+                             kernel="whole call (sos_chunk_kernel state + sos_scan_kernel + "
+                                    "sos_chunk_kernel run)",
+                             traffic=load_traffic(f"sos_f64_{tag}_sosfilt_butter6_pmc")))
+    from oracle import oracle as orc
+    rows = [0, B - 1]
+    ref_fn = orc.filtfilt_f64 if mode == "filtfilt" else orc.sosfilt_f64
+    xs = x[rows].cpu().numpy()
+    ys = y[rows].cpu().numpy()
+    ref = ref_fn(sos_np, xs)
+    out["max_rel_err_vs_reference_order"] = float(np.max(np.abs(ys - ref)) / np.max(np.abs(ref)))
+    if not args.no_cpu_baseline:
+        # the oracle's restatement of the reference loop, one thread, whole rows
+        xh = x[:64].cpu().numpy()
+        t0 = time.perf_counter()
+        done = 0
+        while time.perf_counter() - t0 < args.cpu_budget:  # cycles over the 64 rows
+            ref_fn(sos_np, xh[done % 64])
+            done += 1
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(value=round(done * N / dt / 1e9, 6), unit="Gsamples/s",
+                                   cores=1, kind="port",
+                                   sample=f"{done} whole rows (64 distinct) of the workload, "
+                                          f"1 thread, {dt:.1f} s")
+    return out
+
+
 def run_copy(args):
     """Plain device copy of 1 GiB (2^28 floats): the attainable-bandwidth
     reference and the calibration kernel for the HBM byte counters."""
@@ -411,7 +484,8 @@ def main():
                     help="untimed steps before the timed ones (default: enough for about "
                          "25 ms of work, past the clock transient after a load step)")
     ap.add_argument("--workload", default=HEADLINE,
-                    choices=sorted(DENSE) + ["ragged", "vjp_k255", "resample_2_3", "copy_1gib"])
+                    choices=sorted(DENSE) + ["ragged", "vjp_k255", "resample_2_3",
+                                             "sosfilt_butter6", "copy_1gib"])
     ap.add_argument("--precision", default=None)
     ap.add_argument("--quick", action="store_true",
                     help="skip the side comparisons (and, for ragged, the per-signal launches)")
@@ -440,6 +514,8 @@ def main():
         out = run_copy(args)
     elif args.workload == "resample_2_3":
         out = run_resample(args)
+    elif args.workload == "sosfilt_butter6":
+        out = run_sosfilt(args)
     else:
         out = run_dense(args)
     print(json.dumps(out), flush=True)

@@ -32,6 +32,7 @@
 #ifndef SCIR_HIP_H
 #define SCIR_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #if defined(__GNUC__) || defined(__clang__)
@@ -115,6 +116,29 @@ SCIR_HIP_API int scir_hip_fir_f32_cpu(const float *x, int64_t ldx, const float *
                          int64_t B, int64_t N, float *y, int64_t ldy,
                          int32_t nthreads);
 
+/* ---- device-resident arrays -------------------------------------------------
+ * The HIP arm of scir_gpu::DeviceArray (crates/scir-gpu/src/lib.rs:76-189),
+ * whose to_device only tags host storage: a buffer owns device memory on the
+ * current device until destroyed, so chained FIR and elementwise calls on it
+ * need no host transfers.  Upload/download are synchronous (null stream). */
+typedef struct scir_hip_buffer_s *scir_hip_buffer;
+SCIR_HIP_API int scir_hip_buffer_create(size_t bytes, scir_hip_buffer *buf);
+SCIR_HIP_API int scir_hip_buffer_upload(scir_hip_buffer buf, const void *host, size_t bytes);
+SCIR_HIP_API int scir_hip_buffer_download(scir_hip_buffer buf, void *host, size_t bytes);
+/* Device address of the buffer, for the *_dev entry points (NULL for NULL). */
+SCIR_HIP_API void *scir_hip_buffer_device_ptr(scir_hip_buffer buf);
+/* Waits for the device, then frees. */
+SCIR_HIP_API int scir_hip_buffer_destroy(scir_hip_buffer buf);
+/* Elementwise float32 on device pointers, n elements, enqueued on `stream`:
+ * DeviceArray::add_scalar / mul_scalar / add (crates/scir-gpu/src/lib.rs:206-260);
+ * one IEEE operation per element, so results equal the reference's bit for bit. */
+SCIR_HIP_API int scir_hip_add_scalar_f32_dev(const float *a, float alpha, float *out, int64_t n,
+                                             void *stream);
+SCIR_HIP_API int scir_hip_mul_scalar_f32_dev(const float *a, float alpha, float *out, int64_t n,
+                                             void *stream);
+SCIR_HIP_API int scir_hip_add_f32_dev(const float *a, const float *b, float *out, int64_t n,
+                                      void *stream);
+
 /* ---- polyphase resampling (float64) -----------------------------------------
  * Replaces the reference's CPU-only resample_poly
  * (crates/scir-signal/src/lib.rs:313-362: x upsampled by `up` with zeros,
@@ -146,6 +170,46 @@ SCIR_HIP_API int scir_hip_resample_poly_f64_cpu(const double *x, int64_t ldx, in
                                                 int64_t N, int32_t up, int32_t down,
                                                 const double *h, int32_t K, double *y,
                                                 int64_t ldy, int32_t nthreads);
+
+/* ---- IIR filtering by second-order sections (float64) ----------------------
+ * Replaces the reference's CPU-only sosfilt and filtfilt
+ * (crates/scir-signal/src/lib.rs:260-291), batched over rows.  Each row is
+ * filtered by the Direct Form II Transposed cascade of
+ * crates/scir-iir-filters/src/filter.rs:47-62 from zero state.  sos holds S
+ * sections of six doubles {b0, b1, b2, a0, a1, a2} (the #[repr(C)] layout
+ * of SosCoeffs, and scipy's sos rows); a0 is not read (sections are taken as
+ * normalised, as in the reference).  filtfilt runs the cascade forward, then
+ * backward over that output from zero state (the reference's zero-phase
+ * filter, no padding).  0 <= S <= SCIR_HIP_SOS_MAX_SECTIONS; S = 0 copies.
+ * flags: 0 splits rows into chunks joined through the state transition
+ * (parallel in time; within a few ulps of the reference loop), or
+ * SCIR_HIP_SOS_REFERENCE_ORDER: one chunk per row, every output equal to the
+ * reference loop bit for bit (parallel over rows only). */
+#define SCIR_HIP_SOS_MAX_SECTIONS 16
+#define SCIR_HIP_SOS_REFERENCE_ORDER 1
+/* Device tier: x [rows, ldx >= n], y [rows, ldy >= n] float64 device
+ * pointers (y == x allowed), sos [S * 6] device pointer, on `stream`. */
+SCIR_HIP_API int scir_hip_sosfilt_f64_dev(const double *x, int64_t ldx, int64_t rows, int64_t n,
+                                          const double *sos, int32_t S, double *y, int64_t ldy,
+                                          int32_t flags, void *stream);
+SCIR_HIP_API int scir_hip_filtfilt_f64_dev(const double *x, int64_t ldx, int64_t rows, int64_t n,
+                                           const double *sos, int32_t S, double *y, int64_t ldy,
+                                           int32_t flags, void *stream);
+/* Host tier: host arrays in and out, synchronous (like scir_hip_fir_f32_host). */
+SCIR_HIP_API int scir_hip_sosfilt_f64_host(const double *x, int64_t ldx, int64_t rows, int64_t n,
+                                           const double *sos, int32_t S, double *y, int64_t ldy,
+                                           int32_t flags);
+SCIR_HIP_API int scir_hip_filtfilt_f64_host(const double *x, int64_t ldx, int64_t rows,
+                                            int64_t n, const double *sos, int32_t S, double *y,
+                                            int64_t ldy, int32_t flags);
+/* CPU arm: the reference loop per row (bit-identical to sosfilt / filtfilt),
+ * rows spread over `nthreads` host threads (<= 0: all). */
+SCIR_HIP_API int scir_hip_sosfilt_f64_cpu(const double *x, int64_t ldx, int64_t rows, int64_t n,
+                                          const double *sos, int32_t S, double *y, int64_t ldy,
+                                          int32_t nthreads);
+SCIR_HIP_API int scir_hip_filtfilt_f64_cpu(const double *x, int64_t ldx, int64_t rows, int64_t n,
+                                           const double *sos, int32_t S, double *y, int64_t ldy,
+                                           int32_t nthreads);
 
 /* ---- ragged (CSR) batches -------------------------------------------------
  * No counterpart in the reference (its FIR takes one dense Array2).  Signal s

@@ -30,6 +30,11 @@
  *                        reference's fixture input within the reference test's
  *                        own tolerance, brute-force polyphase sums, impulse
  *                        responses, the reference's output length.
+ *   oracle_sosfilt_f64 / oracle_filtfilt_f64  the reference's own IIR fixtures
+ *                        (crates/scir-iir-filters/test_data, 1e-12), scipy
+ *                        sosfilt on random data, the closed-form impulse
+ *                        response of a resonator section, filtfilt as the
+ *                        reference fixture script's composition.
  *   oracle_fir_vjp_f64   central finite differences of sum(w*y) (exact for a
  *                        bilinear loss up to rounding) and torch float64
  *                        autograd through conv1d.
@@ -191,4 +196,49 @@ int64_t oracle_resample_poly_f64(const double *x, int64_t n, int64_t up, int64_t
     int64_t m = 0;
     for (int64_t idx = off; idx < end; idx += down) y[m++] = work[idx];
     return m;
+}
+
+/* crates/scir-iir-filters/src/filter.rs:47-62 (DirectForm2Transposed::filter)
+ * as driven by crates/scir-signal/src/lib.rs:260-267 (sosfilt): zero state
+ * s1[i] = s2[i] = 0, then for each sample v = x[t] and each section i,
+ *   p = v;  v = b0*p + s1[i];  s1[i] = b1*p - a1*v + s2[i];  s2[i] = b2*p - a2*v
+ * with the expressions evaluated left to right, every operation rounded.
+ * sos holds S rows {b0, b1, b2, a0, a1, a2}; a0 is not read.  `state` must
+ * hold 2*S doubles. */
+void oracle_sosfilt_f64(const double *x, int64_t n, const double *sos, int64_t S, double *state,
+                        double *y)
+{
+    double *s1 = state, *s2 = state + S;
+    for (int64_t i = 0; i < S; ++i) s1[i] = s2[i] = 0.0;
+    for (int64_t t = 0; t < n; ++t) {
+        double output = x[t];
+        for (int64_t i = 0; i < S; ++i) {
+            const double *c = sos + 6 * i;
+            const double prev_output = output;
+            output = c[0] * prev_output + s1[i];
+            s1[i] = c[1] * prev_output - c[4] * output + s2[i];
+            s2[i] = c[2] * prev_output - c[5] * output;
+        }
+        y[t] = output;
+    }
+}
+
+/* crates/scir-signal/src/lib.rs:278-291 (filtfilt): tmp = the cascade over
+ * x from zero state; a fresh cascade over tmp in reverse order gives out;
+ * the result is out reversed.  `tmp` must hold n doubles, `state` 2*S. */
+void oracle_filtfilt_f64(const double *x, int64_t n, const double *sos, int64_t S, double *state,
+                         double *tmp, double *y)
+{
+    oracle_sosfilt_f64(x, n, sos, S, state, tmp);
+    for (int64_t t = 0; t < n / 2; ++t) {  /* tmp.iter().rev() */
+        const double a = tmp[t];
+        tmp[t] = tmp[n - 1 - t];
+        tmp[n - 1 - t] = a;
+    }
+    oracle_sosfilt_f64(tmp, n, sos, S, state, y);
+    for (int64_t t = 0; t < n / 2; ++t) {  /* out.reverse() */
+        const double a = y[t];
+        y[t] = y[n - 1 - t];
+        y[n - 1 - t] = a;
+    }
 }

@@ -67,6 +67,10 @@ def _load():
         lib.oracle_resample_poly_f64.argtypes = [_f64p, _i64, _i64, _i64, _f64p, _i64, _f64p,
                                                  _f64p]
         lib.oracle_resample_poly_f64.restype = _i64
+        lib.oracle_sosfilt_f64.argtypes = [_f64p, _i64, _f64p, _i64, _f64p, _f64p]
+        lib.oracle_sosfilt_f64.restype = None
+        lib.oracle_filtfilt_f64.argtypes = [_f64p, _i64, _f64p, _i64, _f64p, _f64p, _f64p]
+        lib.oracle_filtfilt_f64.restype = None
         _lib = lib
     return _lib
 
@@ -172,6 +176,44 @@ def resample_poly_f64(x, up: int, down: int, h) -> np.ndarray:
     y = np.zeros(max(1, resample_len(n, up, down, K)), np.float64)
     m = _load().oracle_resample_poly_f64(x, n, up, down, h, K, work, y)
     return y[:m]
+
+
+def _sos_args(x, sos):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    sos = np.ascontiguousarray(sos, dtype=np.float64).reshape(-1, 6)
+    return x, sos
+
+
+def sosfilt_f64(sos, x) -> np.ndarray:
+    """The reference's sosfilt (Direct Form II Transposed cascade, zero
+    state, reference operation order) over the last axis of ``x`` (1-D or
+    2-D); ``sos`` is [S, 6] {b0, b1, b2, a0, a1, a2}, a0 not read."""
+    x, sos = _sos_args(x, sos)
+    rows = x.reshape(-1, x.shape[-1]) if x.ndim else x.reshape(1, 1)
+    y = np.empty_like(rows)
+    state = np.zeros(max(1, 2 * sos.shape[0]), np.float64)
+    for r in range(rows.shape[0]):
+        src = np.ascontiguousarray(rows[r])
+        out = np.empty_like(src)
+        _load().oracle_sosfilt_f64(src, src.size, sos, sos.shape[0], state, out)
+        y[r] = out
+    return y.reshape(x.shape)
+
+
+def filtfilt_f64(sos, x) -> np.ndarray:
+    """The reference's filtfilt: the cascade forward, then a fresh cascade
+    over that output backward (no padding), over the last axis of ``x``."""
+    x, sos = _sos_args(x, sos)
+    rows = x.reshape(-1, x.shape[-1])
+    y = np.empty_like(rows)
+    state = np.zeros(max(1, 2 * sos.shape[0]), np.float64)
+    tmp = np.empty(max(1, rows.shape[1]), np.float64)
+    for r in range(rows.shape[0]):
+        src = np.ascontiguousarray(rows[r])
+        out = np.empty_like(src)
+        _load().oracle_filtfilt_f64(src, src.size, sos, sos.shape[0], state, tmp, out)
+        y[r] = out
+    return y.reshape(x.shape)
 
 
 def rel_l2(y, y64) -> float:

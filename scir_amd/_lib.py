@@ -64,6 +64,20 @@ SIGNATURES = {
     "scir_hip_autotune_load": (_I, [_c.c_char_p]),
     "scir_hip_tune_set": (_I, [_c.c_char_p, _I32]),
     "scir_hip_synth_normal_f32": (_I, [_P, _I64, _I64, _I64, _U64, _F, _F, _P]),
+    "scir_hip_sosfilt_f64_dev": (_I, [_P, _I64, _I64, _I64, _P, _I32, _P, _I64, _I32, _P]),
+    "scir_hip_filtfilt_f64_dev": (_I, [_P, _I64, _I64, _I64, _P, _I32, _P, _I64, _I32, _P]),
+    "scir_hip_sosfilt_f64_host": (_I, [_P, _I64, _I64, _I64, _P, _I32, _P, _I64, _I32]),
+    "scir_hip_filtfilt_f64_host": (_I, [_P, _I64, _I64, _I64, _P, _I32, _P, _I64, _I32]),
+    "scir_hip_sosfilt_f64_cpu": (_I, [_P, _I64, _I64, _I64, _P, _I32, _P, _I64, _I32]),
+    "scir_hip_filtfilt_f64_cpu": (_I, [_P, _I64, _I64, _I64, _P, _I32, _P, _I64, _I32]),
+    "scir_hip_buffer_create": (_I, [_c.c_size_t, _c.POINTER(_P)]),
+    "scir_hip_buffer_upload": (_I, [_P, _P, _c.c_size_t]),
+    "scir_hip_buffer_download": (_I, [_P, _P, _c.c_size_t]),
+    "scir_hip_buffer_device_ptr": (_P, [_P]),
+    "scir_hip_buffer_destroy": (_I, [_P]),
+    "scir_hip_add_scalar_f32_dev": (_I, [_P, _F, _P, _I64, _P]),
+    "scir_hip_mul_scalar_f32_dev": (_I, [_P, _F, _P, _I64, _P]),
+    "scir_hip_add_f32_dev": (_I, [_P, _P, _P, _I64, _P]),
     "scir_hip_resample_len": (_I, [_I64, _I32, _I32, _I32, _Pi64]),
     "scir_hip_resample_poly_f64_dev": (_I, [_P, _I64, _I64, _I64, _I32, _I32, _P, _I32, _P,
                                             _I64, _P]),

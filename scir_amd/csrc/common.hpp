@@ -47,6 +47,7 @@ struct DeviceBuffer {
 // streams get their own set.
 struct StreamScratch {
     DeviceBuffer partials;    // per-workgroup dL/db partial sums
+    DeviceBuffer sos_states;  // per-chunk IIR states and the chunk transition
 };
 
 struct DeviceState {

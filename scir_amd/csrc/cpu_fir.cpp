@@ -145,3 +145,79 @@ extern "C" int scir_hip_resample_poly_f64_cpu(const double *x, int64_t ldx, int6
     for (auto &t : pool) t.join();
     return SCIR_HIP_OK;
 }
+
+// CPU arm of sosfilt / filtfilt (crates/scir-signal/src/lib.rs:260-291): the
+// Direct Form II Transposed cascade of crates/scir-iir-filters/src/filter.rs
+// :47-62 per row, in its operation order with separate roundings (this file
+// is compiled with -ffp-contract=off), so the output is bit-identical.
+namespace {
+
+constexpr int kMaxSos = SCIR_HIP_SOS_MAX_SECTIONS;
+
+// Filters positions first, first+step, ... (n of them) of xr into yr from
+// zero state; yr may be xr.
+void sos_row(const double *xr, double *yr, int64_t n, int64_t first, int64_t step,
+             const double *sos, int S)
+{
+    double s1[kMaxSos] = {0}, s2[kMaxSos] = {0};
+    for (int64_t k = 0, p = first; k < n; ++k, p += step) {
+        double v = xr[p];
+        for (int i = 0; i < S; ++i) {
+            const double *c = sos + 6 * i;
+            const double prev = v;
+            v = c[0] * prev + s1[i];
+            s1[i] = c[1] * prev - c[4] * v + s2[i];
+            s2[i] = c[2] * prev - c[5] * v;
+        }
+        yr[p] = v;
+    }
+}
+
+int sos_cpu(const double *x, int64_t ldx, int64_t rows, int64_t N, const double *sos, int32_t S,
+            double *y, int64_t ldy, int32_t nthreads, bool zero_phase)
+{
+    scir::clear_error();
+    if (rows < 0 || N < 0) return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "negative size");
+    if (S < 0 || S > kMaxSos)
+        return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "sections S=%d outside [0, %d]", S,
+                          kMaxSos);
+    if (rows == 0 || N == 0) return SCIR_HIP_OK;
+    if (!x || !y || (S > 0 && !sos))
+        return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "null pointer argument");
+    if (ldx < N || ldy < N)
+        return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row stride smaller than the row length");
+    int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
+    nt = (int)std::min<int64_t>(nt, rows);
+    auto work = [&](int w) {
+        for (int64_t r = w; r < rows; r += nt) {
+            double *yr = y + r * ldy;
+            sos_row(x + r * ldx, yr, N, 0, 1, sos, S);
+            if (zero_phase) sos_row(yr, yr, N, N - 1, -1, sos, S);
+        }
+    };
+    if (nt <= 1) {
+        work(0);
+        return SCIR_HIP_OK;
+    }
+    std::vector<std::thread> pool;
+    pool.reserve(nt);
+    for (int w = 0; w < nt; ++w) pool.emplace_back(work, w);
+    for (auto &t : pool) t.join();
+    return SCIR_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" int scir_hip_sosfilt_f64_cpu(const double *x, int64_t ldx, int64_t rows, int64_t n,
+                                        const double *sos, int32_t S, double *y, int64_t ldy,
+                                        int32_t nthreads)
+{
+    return sos_cpu(x, ldx, rows, n, sos, S, y, ldy, nthreads, false);
+}
+
+extern "C" int scir_hip_filtfilt_f64_cpu(const double *x, int64_t ldx, int64_t rows, int64_t n,
+                                         const double *sos, int32_t S, double *y, int64_t ldy,
+                                         int32_t nthreads)
+{
+    return sos_cpu(x, ldx, rows, n, sos, S, y, ldy, nthreads, true);
+}

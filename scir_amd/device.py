@@ -90,3 +90,25 @@ def synth_normal(B: int, N: int, seed: int, outlier_frac: float = 0.0,
         _vp(out), out.stride(0), B, N, ctypes.c_uint64(seed & (2**64 - 1)),
         float(outlier_frac), float(outlier_scale), _stream_ptr(out)))
     return out
+
+
+def sosfilt(x: torch.Tensor, sos: torch.Tensor, reference_order: bool = False,
+            zero_phase: bool = False, out: torch.Tensor | None = None):
+    """IIR cascade (``sosfilt``, or ``filtfilt`` with ``zero_phase``) over
+    each row of a float64 tensor; ``sos`` is a contiguous [S, 6] float64
+    tensor {b0, b1, b2, a0, a1, a2} on the same device (a0 not read).
+    ``out`` may be ``x`` (in place)."""
+    if x.dtype != torch.float64 or not x.is_cuda or x.dim() != 2 or x.stride(1) != 1:
+        raise TypeError("x must be a 2-D float64 HIP tensor with contiguous rows")
+    if (sos.dtype != torch.float64 or not sos.is_cuda or sos.dim() != 2 or sos.shape[1] != 6
+            or not sos.is_contiguous()):
+        raise TypeError("sos must be a contiguous [S, 6] float64 HIP tensor")
+    rows, n = x.shape
+    if out is None:
+        out = torch.empty((rows, n), dtype=torch.float64, device=x.device)
+    if out.dtype != torch.float64 or tuple(out.shape) != (rows, n) or out.stride(1) != 1:
+        raise ValueError("out must be a float64 tensor of the shape of x")
+    fn = _lib.lib().scir_hip_filtfilt_f64_dev if zero_phase else _lib.lib().scir_hip_sosfilt_f64_dev
+    _lib.check(fn(_vp(x), max(x.stride(0), n), rows, n, _vp(sos), sos.shape[0], _vp(out),
+                  max(out.stride(0), n), 1 if reference_order else 0, _stream_ptr(x)))
+    return out

@@ -142,3 +142,184 @@ def fir1d_batched_f32_auto(x, taps, device: Device) -> np.ndarray:
                 return fir1d_batched_f32(x, taps)
             raise
     raise ValueError(f"unknown device {device!r}")
+
+
+class DeviceArray:
+    """``scir_gpu::DeviceArray`` (crates/scir-gpu/src/lib.rs:76-260) with a
+    device-resident HIP arm.
+
+    The reference keeps host storage whatever the device and ``to_device``
+    only tags it.  Here ``to_device(Device.Hip)`` copies the data into a
+    library-owned device buffer and later operations on the array --
+    ``add_scalar``, ``mul_scalar``, ``add`` and :func:`fir1d_batched_f32_resident`
+    -- run on the device without host transfers; ``to_cpu_vec`` and
+    ``to_device(Device.Cpu)`` copy back.  Arrays on the HIP device hold
+    float32 (the dtype of every HIP operation here).
+
+    >>> import numpy as np
+    >>> a = DeviceArray.from_cpu_slice([2, 2], DType.F32, [1.0, 2.0, 3.0, 4.0])
+    >>> a.shape(), a.device()
+    ([2, 2], <Device.Cpu: 'cpu'>)
+    >>> a.add_scalar(1.0).to_cpu_vec().tolist()
+    [2.0, 3.0, 4.0, 5.0]
+    """
+
+    def __init__(self, shape, dtype: DType, host: np.ndarray):
+        self._shape = [int(s) for s in shape]
+        self._dtype = dtype
+        self._device = Device.Cpu
+        self._host = host          # 1-D numpy storage when on the CPU
+        self._buf = None           # scir_hip_buffer when on the HIP device
+
+    @staticmethod
+    def _np_dtype(dtype: DType):
+        return np.float32 if dtype is DType.F32 else np.float64
+
+    @classmethod
+    def from_cpu_slice(cls, shape, dtype: DType, data) -> "DeviceArray":
+        """Copy ``data`` (length = product of ``shape``) into a CPU array."""
+        arr = np.array(data, dtype=cls._np_dtype(dtype)).ravel()
+        if int(np.prod(shape, dtype=np.int64)) != arr.size:
+            raise ValueError("shape does not match the data length")
+        return cls(shape, dtype, arr)
+
+    def shape(self):
+        return list(self._shape)
+
+    def dtype(self) -> DType:
+        return self._dtype
+
+    def device(self) -> Device:
+        return self._device
+
+    def _size(self) -> int:
+        return int(np.prod(self._shape, dtype=np.int64))
+
+    def _nbytes(self) -> int:
+        return self._size() * np.dtype(self._np_dtype(self._dtype)).itemsize
+
+    def _dev_ptr(self):
+        return ctypes.c_void_p(_lib.lib().scir_hip_buffer_device_ptr(self._buf))
+
+    def to_cpu_vec(self) -> np.ndarray:
+        """The elements as a new 1-D host array."""
+        if self._device is Device.Cpu:
+            return self._host.copy()
+        out = np.empty(self._size(), dtype=self._np_dtype(self._dtype))
+        _check_gpu(_lib.lib().scir_hip_buffer_download(self._buf, _ptr(out), self._nbytes()))
+        return out
+
+    def to_device(self, device: Device) -> None:
+        """Move the array (``Device.Hip``: allocate and copy; ``Device.Cpu``:
+        copy back and free).  Raises ``GpuError`` on failure."""
+        if device is self._device:
+            return
+        if device is Device.Cpu:
+            self._host = self.to_cpu_vec()
+            self._release()
+            self._device = Device.Cpu
+            return
+        if device is not Device.Hip:
+            raise ValueError(f"unknown device {device!r}")
+        if self._dtype is not DType.F32:
+            raise BackendUnavailable("arrays on the HIP device hold float32")
+        try:
+            L = _lib.lib()
+        except _lib.LibraryMissing as e:
+            raise BackendUnavailable(str(e)) from None
+        buf = ctypes.c_void_p()
+        _check_gpu(L.scir_hip_buffer_create(self._nbytes(), ctypes.byref(buf)))
+        status = L.scir_hip_buffer_upload(buf, _ptr(self._host), self._nbytes())
+        if status != _lib.OK:
+            L.scir_hip_buffer_destroy(buf)
+            _check_gpu(status)
+        self._buf, self._host, self._device = buf, None, Device.Hip
+
+    def _release(self):
+        if self._buf is not None:
+            _lib.lib().scir_hip_buffer_destroy(self._buf)
+            self._buf = None
+
+    def __del__(self):
+        try:
+            self._release()
+        except Exception:
+            pass
+
+    def _like_on_device(self) -> "DeviceArray":
+        out = DeviceArray(self._shape, self._dtype, None)
+        buf = ctypes.c_void_p()
+        _check_gpu(_lib.lib().scir_hip_buffer_create(self._nbytes(), ctypes.byref(buf)))
+        out._buf, out._device = buf, Device.Hip
+        return out
+
+    def _elementwise(self, name, alpha=None, other=None) -> "DeviceArray":
+        if self._device is Device.Cpu:
+            a = self._host
+            t = a.dtype.type
+            res = {"add_scalar": lambda: a + t(alpha), "mul_scalar": lambda: a * t(alpha),
+                   "add": lambda: a + other._host}[name]()
+            return DeviceArray(self._shape, self._dtype, res.astype(a.dtype, copy=False))
+        out = self._like_on_device()
+        L = _lib.lib()
+        n = self._size()
+        if name == "add_scalar":
+            st = L.scir_hip_add_scalar_f32_dev(self._dev_ptr(), float(alpha), out._dev_ptr(), n, None)
+        elif name == "mul_scalar":
+            st = L.scir_hip_mul_scalar_f32_dev(self._dev_ptr(), float(alpha), out._dev_ptr(), n, None)
+        else:
+            st = L.scir_hip_add_f32_dev(self._dev_ptr(), other._dev_ptr(), out._dev_ptr(), n, None)
+        _check_gpu(st)
+        return out
+
+    def add_scalar(self, alpha) -> "DeviceArray":
+        """Elementwise ``x + alpha`` on the array's device (lib.rs:206-221)."""
+        return self._elementwise("add_scalar", alpha=alpha)
+
+    def mul_scalar(self, alpha) -> "DeviceArray":
+        """Elementwise ``x * alpha`` on the array's device (lib.rs:223-243)."""
+        return self._elementwise("mul_scalar", alpha=alpha)
+
+    def add(self, other: "DeviceArray") -> "DeviceArray":
+        """Elementwise sum; ``ShapeMismatch`` for other shapes or devices
+        (lib.rs:245-260)."""
+        if other._shape != self._shape or other._device is not self._device:
+            raise ShapeMismatch("add needs arrays of one shape on one device")
+        return self._elementwise("add", other=other)
+
+    # The reference's ``*_auto`` names (lib.rs:262-380) dispatch on the
+    # array's device, as the methods above do.
+    add_scalar_auto = add_scalar
+    mul_scalar_auto = mul_scalar
+    add_auto = add
+
+
+def _check_gpu(status: int) -> None:
+    if status == _lib.OK:
+        return
+    msg = _lib.lib().scir_hip_last_error().decode()
+    if status == _lib.ERR_INVALID_ARGUMENT:
+        raise ShapeMismatch(msg)
+    raise BackendUnavailable(msg)
+
+
+def fir1d_batched_f32_resident(x: DeviceArray, taps, precision="f32") -> DeviceArray:
+    """Batched FIR of a ``[batch, n]`` array on the HIP device; the result
+    stays on the device (no host transfer of x or y).  ``taps`` in the
+    reference's order."""
+    if x.device() is not Device.Hip or len(x.shape()) != 2:
+        raise ShapeMismatch("x must be a 2-D DeviceArray on Device.Hip")
+    B, N = x.shape()
+    b = _as_b(taps)
+    L = _lib.lib()
+    out = x._like_on_device()
+    bbuf = ctypes.c_void_p()
+    _check_gpu(L.scir_hip_buffer_create(max(1, b.nbytes), ctypes.byref(bbuf)))
+    try:
+        _check_gpu(L.scir_hip_buffer_upload(bbuf, _ptr(b), b.nbytes))
+        _check_gpu(L.scir_hip_fir_f32_dev(x._dev_ptr(), N, ctypes.c_void_p(
+            L.scir_hip_buffer_device_ptr(bbuf)), b.size, B, N, out._dev_ptr(), N,
+            _lib.precision_code(precision), None))
+    finally:
+        L.scir_hip_buffer_destroy(bbuf)  # waits for the device first
+    return out

@@ -1,0 +1,133 @@
+"""sosfilt / filtfilt on the GPU against the oracle (the reference loop).
+
+reference_order: every output equal to the reference loop bit for bit.
+Chunked (default): the first 256 outputs of each row bit for bit (the first
+chunk starts from zero state like the reference), every output within
+1e-12 of the row's largest |y| (the chunk joins carry the rounding of the
+state transition; measured errors are a few 1e-16)."""
+import pathlib
+
+import numpy as np
+import pytest
+import torch
+from scipy import signal as ss
+
+from oracle import oracle as orc
+from scir_amd import device, signal
+from scir_amd.gpu import Device
+
+pytestmark = pytest.mark.gpu
+GOLD = pathlib.Path(__file__).resolve().parent / "golden"
+REL_TOL = 1e-12
+
+
+def _sos(S, seed=0):
+    if S == 0:
+        return np.zeros((0, 6))
+    base = [ss.butter(2 * S, 0.15, output="sos"),
+            ss.cheby1(2 * S, 0.5, 0.3, output="sos"),
+            ss.butter(S, [0.1, 0.35], "bandpass", output="sos")[:S]]
+    return np.ascontiguousarray(base[seed % 3][:S])
+
+
+def _check(y, ref, exact_prefix):
+    assert y.shape == ref.shape
+    if exact_prefix:
+        k = min(256, ref.shape[1])
+        assert np.array_equal(y[:, :k], ref[:, :k])
+    scale = np.max(np.abs(ref), axis=1, keepdims=True) + 1e-300
+    err = np.max(np.abs(y - ref) / scale)
+    assert err <= REL_TOL, err
+
+
+@pytest.mark.parametrize("S", [1, 2, 3, 4, 5, 6, 7, 8, 9, 16])
+@pytest.mark.parametrize("shape", [(3, 1), (2, 17), (4, 1000), (3, 4133), (1, 70001)])
+def test_reference_order_bitwise(S, shape):
+    print(f"sosfilt reference_order S={S} shape={shape}", flush=True)
+    rng = np.random.default_rng(S * 7 + shape[1])
+    x = rng.standard_normal(shape)
+    sos = _sos(S, S)
+    y = signal.sosfilt_batched(sos, x, Device.Hip, reference_order=True)
+    assert np.array_equal(y, orc.sosfilt_f64(sos, x))
+    yf = signal.filtfilt_batched(sos, x, Device.Hip, reference_order=True)
+    assert np.array_equal(yf, orc.filtfilt_f64(sos, x))
+
+
+@pytest.mark.parametrize("S", [1, 2, 3, 4, 8, 9, 16])
+@pytest.mark.parametrize("shape", [(2, 17), (4, 1000), (3, 4133), (1, 70001), (64, 9000)])
+def test_chunked_close(S, shape):
+    print(f"sosfilt chunked S={S} shape={shape}", flush=True)
+    rng = np.random.default_rng(S * 11 + shape[1])
+    x = rng.standard_normal(shape)
+    sos = _sos(S, S + 1)
+    _check(signal.sosfilt_batched(sos, x, Device.Hip), orc.sosfilt_f64(sos, x), True)
+    _check(signal.filtfilt_batched(sos, x, Device.Hip), orc.filtfilt_f64(sos, x), False)
+
+
+@pytest.mark.parametrize("design", ["low_pass", "high_pass", "band_pass", "band_stop"])
+def test_reference_iir_fixtures(design):
+    # the reference's own DF2T fixtures at its tolerance (1e-12 absolute)
+    x = np.load(GOLD / "iir_bridge.npy")[1]
+    sos = np.load(GOLD / f"iir_{design}_sos.npy")
+    want = np.load(GOLD / f"iir_{design}_expected.npy")
+    for ref_order in (False, True):
+        y = signal.sosfilt_batched(sos, x[None, :], Device.Hip, reference_order=ref_order)[0]
+        assert np.max(np.abs(y - want)) <= 1e-12
+    assert np.array_equal(signal.sosfilt(sos, x, Device.Hip)[:256],
+                          orc.sosfilt_f64(sos, x)[:256])
+
+
+def test_signal_fixtures_and_edges():
+    x = np.load(GOLD / "resample_x.npy")
+    for name in ["butter", "cheby", "bessel", "notch"]:
+        sos = np.load(GOLD / f"sos_{name}.npy")
+        assert np.array_equal(signal.sosfilt(sos, x, Device.Hip),
+                              np.load(GOLD / f"sosfilt_{name}.npy"))
+    sos = np.load(GOLD / "sos_butter.npy")
+    assert np.array_equal(signal.filtfilt(sos, x, Device.Hip), np.load(GOLD / "filtfilt_butter.npy"))
+    # no sections: the input passes through; empty inputs
+    xr = np.random.default_rng(0).standard_normal((3, 500))
+    assert np.array_equal(signal.sosfilt_batched(np.zeros((0, 6)), xr, Device.Hip), xr)
+    assert signal.sosfilt(sos, np.zeros(0), Device.Hip).shape == (0,)
+    assert signal.sosfilt_batched(sos, np.zeros((0, 10)), Device.Hip).shape == (0, 10)
+
+
+def test_device_tier_strided_inplace_unaligned():
+    print("sosfilt device tier", flush=True)
+    rng = np.random.default_rng(9)
+    sos_np = _sos(3)
+    sos = torch.tensor(sos_np, device="cuda:0")
+    big = torch.tensor(rng.standard_normal((5, 3001)), device="cuda:0")
+    x = big[:, :2999]                      # row stride 3001 (odd: scalar loads)
+    ref = orc.sosfilt_f64(sos_np, x.cpu().numpy())
+    y = device.sosfilt(x, sos)
+    torch.cuda.synchronize()
+    _check(y.cpu().numpy(), ref, True)
+    # in place, aligned rows
+    z = torch.tensor(rng.standard_normal((4, 4096)), device="cuda:0")
+    zref = orc.filtfilt_f64(sos_np, z.cpu().numpy())
+    device.sosfilt(z, sos, zero_phase=True, out=z)
+    torch.cuda.synchronize()
+    _check(z.cpu().numpy(), zref, False)
+    zr = torch.tensor(rng.standard_normal((4, 4096)), device="cuda:0")
+    zrref = orc.sosfilt_f64(sos_np, zr.cpu().numpy())
+    device.sosfilt(zr, sos, reference_order=True, out=zr)
+    torch.cuda.synchronize()
+    assert np.array_equal(zr.cpu().numpy(), zrref)
+
+
+def test_full_size_rows():
+    # the benchmark shape: a sample of rows against the oracle
+    print("sosfilt full size", flush=True)
+    rows, n = 4096, 65536
+    gen = torch.Generator(device="cuda:0").manual_seed(3)
+    x = torch.randn((rows, n), generator=gen, dtype=torch.float64, device="cuda:0")
+    sos_np = ss.butter(6, 0.15, output="sos")
+    sos = torch.tensor(sos_np, device="cuda:0")
+    y = device.sosfilt(x, sos)
+    torch.cuda.synchronize()
+    for r in (0, 1, 2047, 4095):
+        ref = orc.sosfilt_f64(sos_np, x[r].cpu().numpy())
+        _check(y[r].cpu().numpy()[None, :], ref[None, :], True)
+    del x, y
+    torch.cuda.empty_cache()

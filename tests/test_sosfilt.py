@@ -1,0 +1,106 @@
+"""sosfilt / filtfilt on the CPU: the oracle pinned against the reference's
+own IIR fixtures, scipy and closed forms, and the library's CPU arm against
+the oracle (bit for bit)."""
+import pathlib
+
+import numpy as np
+import pytest
+from scipy import signal as ss
+
+from oracle import oracle as orc
+from scir_amd import signal
+from scir_amd.gpu import Device, ShapeMismatch
+
+GOLD = pathlib.Path(__file__).resolve().parent / "golden"
+DESIGNS = ["low_pass", "high_pass", "band_pass", "band_stop"]
+SIG = ["butter", "cheby", "bessel", "butter_hp", "cheby_hp", "bessel_hp", "notch"]
+
+
+@pytest.mark.parametrize("design", DESIGNS)
+def test_oracle_matches_reference_iir_fixtures(design):
+    # crates/scir-iir-filters/src/filter.rs:124-262: DF2T output vs the
+    # expected files at 1e-12 (the reference test's own tolerance)
+    x = np.load(GOLD / "iir_bridge.npy")[1]
+    sos = np.load(GOLD / f"iir_{design}_sos.npy")
+    want = np.load(GOLD / f"iir_{design}_expected.npy")
+    assert np.max(np.abs(orc.sosfilt_f64(sos, x) - want)) <= 1e-12
+
+
+@pytest.mark.parametrize("name", SIG)
+def test_oracle_matches_signal_fixtures(name):
+    # crates/scir-signal/src/lib.rs:410-560 compare at atol = rtol = 1e-6;
+    # the oracle reproduces scipy's sosfilt bit for bit
+    x = np.load(GOLD / "resample_x.npy")
+    sos = np.load(GOLD / f"sos_{name}.npy")
+    assert np.array_equal(orc.sosfilt_f64(sos, x), np.load(GOLD / f"sosfilt_{name}.npy"))
+
+
+def test_oracle_filtfilt_fixture():
+    x = np.load(GOLD / "resample_x.npy")
+    sos = np.load(GOLD / "sos_butter.npy")
+    assert np.array_equal(orc.filtfilt_f64(sos, x), np.load(GOLD / "filtfilt_butter.npy"))
+
+
+def test_oracle_matches_scipy_random():
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((3, 2000)) * np.array([[1.0], [1e3], [1e-3]])
+    for sos in (ss.butter(6, 0.15, output="sos"), ss.cheby1(8, 0.5, [0.1, 0.4], "bandpass",
+                                                            output="sos")):
+        assert np.array_equal(orc.sosfilt_f64(sos, x), ss.sosfilt(sos, x, axis=-1))
+
+
+def test_oracle_resonator_closed_form():
+    # b = [1, 0, 0], a = [1, -2 r cos(th), r^2]: h[n] = r^n sin((n+1) th) / sin(th)
+    r, th = 0.97, 0.3
+    sos = np.array([[1.0, 0.0, 0.0, 1.0, -2 * r * np.cos(th), r * r]])
+    n = np.arange(400)
+    imp = np.zeros(400)
+    imp[0] = 1.0
+    want = r ** n * np.sin((n + 1) * th) / np.sin(th)
+    assert np.max(np.abs(orc.sosfilt_f64(sos, imp) - want)) <= 1e-13
+
+
+def test_oracle_ignores_a0():
+    sos = ss.butter(4, 0.2, output="sos")
+    x = np.random.default_rng(1).standard_normal(300)
+    sos2 = sos.copy()
+    sos2[:, 3] = 7.0
+    assert np.array_equal(orc.sosfilt_f64(sos, x), orc.sosfilt_f64(sos2, x))
+
+
+def test_oracle_plants():
+    # a wrong sign on a1, or a dropped b2 term, fails the fixture check
+    x = np.load(GOLD / "iir_bridge.npy")[1]
+    sos = np.load(GOLD / "iir_band_pass_sos.npy")
+    want = np.load(GOLD / "iir_band_pass_expected.npy")
+    for idx in ((0, 4), (0, 2)):
+        bad = sos.copy()
+        bad[idx] = -bad[idx] if idx[1] == 4 else 0.0
+        err = np.abs(orc.sosfilt_f64(bad, x) - want)
+        assert not np.all(err <= 1e-12), idx
+
+
+@pytest.mark.parametrize("S", [0, 1, 3, 8, 16])
+def test_cpu_arm_bitwise(S):
+    rng = np.random.default_rng(S)
+    sos = np.tile(ss.butter(2, 0.3, output="sos"), (max(S, 1), 1))[:S]
+    x = rng.standard_normal((3, 777))
+    y = signal.sosfilt_batched(sos, x, Device.Cpu)
+    if S == 0:
+        assert np.array_equal(y, x)
+    else:
+        assert np.array_equal(y, orc.sosfilt_f64(sos, x))
+        assert np.array_equal(signal.filtfilt_batched(sos, x, Device.Cpu),
+                              orc.filtfilt_f64(sos, x))
+
+
+def test_mirror_contract():
+    sos = np.load(GOLD / "sos_butter.npy")
+    x = np.load(GOLD / "resample_x.npy")
+    assert signal.sosfilt(sos, x).shape == x.shape
+    assert signal.filtfilt(sos, x).shape == x.shape
+    assert signal.sosfilt(sos, np.zeros(0)).shape == (0,)
+    with pytest.raises(ShapeMismatch):
+        signal.sosfilt_batched(np.zeros((2, 5)), np.zeros((1, 4)))
+    with pytest.raises(ShapeMismatch):
+        signal.sosfilt_batched(np.zeros((17, 6)), np.zeros((1, 4)))
