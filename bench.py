@@ -388,10 +388,10 @@ def run_resample(args):
 def run_sosfilt(args):
     """Batched float64 IIR cascade (the reference's sosfilt benchmark filter,
     crates/scir-signal/benches/signal.rs:8-18, butter(6, 0.15) = 3 sections) on
-    [4096, 65536].  The call is three launches (chunk states, chunk scan,
-    filtering pass); its algorithmic HBM bytes are 8 per input sample read and
-    8 per output written (16 per sample; the chunked method reads x twice,
-    24 bytes a sample at the counters)."""
+    [4096, 65536].  With 4096 rows each row is one wave's chunk: the call is a
+    tiny setup launch and one filtering pass (sos_wave_kernel); its
+    algorithmic HBM bytes are 8 per input sample read and 8 per output
+    written."""
     import ctypes
     import torch
     from scipy import signal as ss
@@ -429,8 +429,7 @@ def run_sosfilt(args):
                wall_s=round(wall, 3),
                roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                              frac=round(gbs / HBM_PEAK_GBS, 4),
-                             kernel="whole call (sos_chunk_kernel state + sos_scan_kernel + "
-                                    "sos_chunk_kernel run)",
+                             kernel="whole call (sos_setup_kernel + sos_wave_kernel)",
                              traffic=load_traffic(f"sos_f64_{tag}_sosfilt_butter6_pmc")))
     from oracle import oracle as orc
     rows = [0, B - 1]

@@ -216,6 +216,36 @@ def filtfilt_f64(sos, x) -> np.ndarray:
     return y.reshape(x.shape)
 
 
+def sosfilt_longdouble(sos, x, zero_phase: bool = False) -> np.ndarray:
+    """The same cascade in x87 extended precision (numpy longdouble, 64-bit
+    mantissa) over the rows of a 2-D ``x``: the near-exact result against
+    which the float64 reference loop's own rounding error is measured (for
+    ill-conditioned filters that error is far above one ulp)."""
+    sos = np.asarray(sos, dtype=np.longdouble).reshape(-1, 6)
+    x = np.asarray(x, dtype=np.longdouble)
+    rows, n = x.shape
+
+    def run(src):
+        s1 = np.zeros((rows, sos.shape[0]), np.longdouble)
+        s2 = np.zeros_like(s1)
+        out = np.empty_like(src)
+        for t in range(n):
+            v = src[:, t]
+            for i in range(sos.shape[0]):
+                b0, b1, b2, _, a1, a2 = sos[i]
+                p = v
+                v = b0 * p + s1[:, i]
+                s1[:, i] = b1 * p - a1 * v + s2[:, i]
+                s2[:, i] = b2 * p - a2 * v
+            out[:, t] = v
+        return out
+
+    y = run(x)
+    if zero_phase:
+        y = run(y[:, ::-1].copy())[:, ::-1]
+    return y
+
+
 def rel_l2(y, y64) -> float:
     """``||y - y64||_2 / ||y64||_2`` computed in float64."""
     y = np.asarray(y, np.float64)

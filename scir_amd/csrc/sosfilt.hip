@@ -9,26 +9,30 @@
 //                        s2[i] = b2*p - a2*v
 // (a0 is not read: sections are taken as normalised, as in the reference).
 //
-// The recursion is sequential in time, so a row is cut into chunks of L
-// samples and the filter's linearity joins them.  With the state
-// z = (s1[0], s2[0], ..., s1[S-1], s2[S-1]) after a chunk,
-//   z_end = T z_start + e,   T = A^L,  e = end state of the chunk from z = 0,
-// where A is the cascade's zero-input state transition.  Three launches:
-//   1. sos_chunk_kernel<STATE>: one lane per chunk runs it from zero state
-//      and keeps e (x read once); one extra workgroup runs L zero-input steps
-//      from each unit state, which gives the columns of T.
-//   2. sos_scan_kernel: per row, an exclusive scan over the chunks,
-//      z_c = T z_{c-1} + e_{c-1} (Kogge-Stone over a wave with T^(2^k)).
-//   3. sos_chunk_kernel<RUN>: one lane per chunk runs it again from z_c and
-//      writes y (x read again, y written once).
-// Every sample step is the reference's arithmetic in its order (no fused
-// multiply-adds), so chunk 0 of each row, and with one chunk per row
-// (SCIR_HIP_SOS_REFERENCE_ORDER) every output, equals the reference's loop
-// bit for bit; later chunks differ only by the rounding of z_c.
+// The recursion is sequential in time; linearity parallelises it.  With the
+// cascade state z = (s1[0], s2[0], ..., s1[S-1], s2[S-1]) and its zero-input
+// transition A, a stretch of n samples maps
+//   z_end = A^n z_start + e,   y = y0 + C z_start,
+// where e and y0 are the stretch's end state and outputs from zero state and
+// C holds the zero-input outputs from unit states.
 //
-// A lane streams its chunk: sixteen samples (128 bytes) per step with
-// 16-byte vector loads, the next group loaded while the current one is
-// filtered.  Lanes of a wave hold consecutive chunks of one row.
+// sos_wave_kernel: one wave filters one chunk of a row, 1024 samples per
+// step (16 consecutive samples a lane, so the wave reads and writes 8 KB of
+// contiguous memory per step).  Each lane runs its 16 samples from zero
+// state (the reference's arithmetic), the wave scans the lanes' end states
+// (Kogge-Stone with (A^16)^(2^k)) to get every lane's start state, and each
+// lane adds C z_start to its outputs; the last lane's state carries into the
+// next step.  With enough rows each row is one chunk: one pass, x read and
+// y written once.  With few rows, rows are cut into chunks of L samples:
+//   1. sos_wave_kernel<STATE>: each chunk's end state from zero state,
+//   2. sos_scan_kernel: per row, z_c = A^L z_{c-1} + e_{c-1},
+//   3. sos_wave_kernel<RUN>: each chunk from z_c.
+// sos_setup_kernel computes C, (A^16)^(2^k) and A^L first.  The first 16
+// outputs of every chunk run from its exact start state (the reference's
+// arithmetic, bit for bit in chunk 0); the others carry the rounding of the
+// joins (measured ~1e-15 relative).  SCIR_HIP_SOS_REFERENCE_ORDER, and rows
+// of at most 1024 samples, use sos_row_kernel instead: one lane a row, the
+// reference loop exactly.
 #include "common.hpp"
 
 #include <algorithm>
@@ -37,8 +41,10 @@ namespace scir {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kGroup = 16;        // samples per streamed group
-constexpr int kMaxSections = 16;  // SCIR_HIP_SOS_MAX_SECTIONS
+constexpr int kGroup = 16;             // samples a lane filters per step
+constexpr int kStep = 64 * kGroup;     // samples a wave filters per step
+constexpr int kMaxSections = 16;       // SCIR_HIP_SOS_MAX_SECTIONS
+constexpr int kPowers = 6;             // (A^16)^(2^k), k < 6: lane offsets 1..32
 
 enum { kState = 0, kRun = 1 };
 
@@ -86,151 +92,375 @@ __device__ __forceinline__ double df2t_step(double v, double (&s1)[SMAX], double
     return v;
 }
 
-__device__ __forceinline__ void load_group(double (&g)[kGroup], const double *p,
-                                           bool vec)
+// Layout of the matrices sos_setup_kernel writes (d = 2S):
+//   C [16][d]            zero-input output k from unit state j
+//   P [kPowers][d][d]    (A^16)^(2^k)
+//   T [d][d]             A^L (the chunk transition; when there are chunks)
+__host__ __device__ constexpr int64_t mats_c(int d) { return 0; }
+__host__ __device__ constexpr int64_t mats_p(int d) { return (int64_t)kGroup * d; }
+__host__ __device__ constexpr int64_t mats_t(int d)
 {
-    if (vec) {
+    return (int64_t)kGroup * d + (int64_t)kPowers * d * d;
+}
+__host__ __device__ constexpr int64_t mats_size(int d) { return mats_t(d) + (int64_t)d * d; }
+
+// One workgroup.  q = L/16 when a chunk transition is needed, else 0.
+template <int SMAX>
+__global__ __launch_bounds__(kThreads) void sos_setup_kernel(const double *__restrict__ sos, int S,
+                                                             int64_t q, double *__restrict__ mats)
+{
+    constexpr int D = 2 * SMAX;
+    __shared__ double base[D][D], res[D][D], tpr[D][D], tsq[D][D];
+    const int d = 2 * S;
+    const int tid = threadIdx.x;
+    if (tid < d) {
+        // 16 zero-input steps from unit state tid: C's column and A^16's
+        Coefs<SMAX> cf;
+        load_coefs(cf, sos, S);
+        double s1[SMAX], s2[SMAX];
+#pragma unroll
+        for (int i = 0; i < SMAX; ++i) {
+            s1[i] = (2 * i == tid) ? 1.0 : 0.0;
+            s2[i] = (2 * i + 1 == tid) ? 1.0 : 0.0;
+        }
+        for (int k = 0; k < kGroup; ++k)
+            mats[mats_c(d) + k * d + tid] = df2t_step<SMAX>(0.0, s1, s2, cf, S);
+#pragma unroll
+        for (int i = 0; i < SMAX; ++i) {
+            if (i < S) {
+                base[2 * i][tid] = s1[i];
+                base[2 * i + 1][tid] = s2[i];
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < d * d; e += kThreads) {
+        res[e / d][e % d] = (e / d == e % d) ? 1.0 : 0.0;
+        mats[mats_p(d) + e] = base[e / d][e % d];
+    }
+    __syncthreads();
+    // base runs through (A^16)^(2^k); res collects A^L = (A^16)^q bit by bit.
+    for (int k = 0;; ++k) {
+        const bool bit = q > 0 && ((q >> k) & 1);
+        const bool more = (k + 1 < kPowers) || ((q >> (k + 1)) > 0);
+        for (int e = tid; e < d * d; e += kThreads) {
+            const int i = e / d, j = e % d;
+            double pr = 0.0, sq = 0.0;
+            for (int m = 0; m < d; ++m) {
+                pr = fma(res[i][m], base[m][j], pr);
+                sq = fma(base[i][m], base[m][j], sq);
+            }
+            tpr[i][j] = pr;
+            tsq[i][j] = sq;
+        }
+        __syncthreads();
+        for (int e = tid; e < d * d; e += kThreads) {
+            const int i = e / d, j = e % d;
+            if (bit) res[i][j] = tpr[i][j];
+            if (more) base[i][j] = tsq[i][j];
+            if (more && k + 1 < kPowers) mats[mats_p(d) + (int64_t)(k + 1) * d * d + e] = tsq[i][j];
+        }
+        __syncthreads();
+        if (!more) break;
+    }
+    if (q > 0)
+        for (int e = tid; e < d * d; e += kThreads) mats[mats_t(d) + e] = res[e / d][e % d];
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Inclusive scan over the wave of the lanes' end states: lane l ends with
+// sum_{j <= l} (A^16)^(l-j) e_j.
+template <int D>
+__device__ __forceinline__ void wave_scan(double (&e)[D], const double (*pw)[D][D], int lane)
+{
+#pragma unroll 1
+    for (int k = 0; k < kPowers; ++k) {
+        const int off = 1 << k;
+        double w[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) w[i] = __shfl_up(e[i], off, 64);
+        if (lane >= off) {
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                double acc = e[i];
+#pragma unroll
+                for (int m = 0; m < D; ++m) acc = fma(pw[k][i][m], w[m], acc);
+                e[i] = acc;
+            }
+        }
+    }
+}
+
+// Loads a lane's group (cnt samples from xr[lo]); missing samples are 0.
+__device__ __forceinline__ void load_lane(double (&v)[kGroup], const double *xr, int64_t lo,
+                                          int64_t cnt, bool vec)
+{
+    if (vec && cnt == kGroup) {
 #pragma unroll
         for (int k = 0; k < kGroup / 2; ++k) {
-            const double2 v = *reinterpret_cast<const double2 *>(p + 2 * k);
-            g[2 * k] = v.x;
-            g[2 * k + 1] = v.y;
+            const double2 t = *reinterpret_cast<const double2 *>(xr + lo + 2 * k);
+            v[2 * k] = t.x;
+            v[2 * k + 1] = t.y;
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < kGroup; ++k) g[k] = p[k];
+        for (int k = 0; k < kGroup; ++k) v[k] = (k < cnt) ? xr[lo + k] : 0.0;
     }
 }
 
-__device__ __forceinline__ void store_group(double *p, const double (&g)[kGroup],
-                                            bool vec)
+// Group gidx (in time order) of the chunk [p0, p1): positions [lo, lo+cnt).
+// Back to front, group 0 is the one holding p1-1 and only it can be short.
+template <bool REV>
+__device__ __forceinline__ void group_span(int64_t gidx, int64_t ngroups, int64_t p0, int64_t p1,
+                                           int64_t *lo, int64_t *cnt)
 {
-    if (vec) {
-#pragma unroll
-        for (int k = 0; k < kGroup / 2; ++k)
-            *reinterpret_cast<double2 *>(p + 2 * k) = make_double2(g[2 * k], g[2 * k + 1]);
+    if (gidx >= ngroups) {
+        *lo = p0;
+        *cnt = 0;
+    } else if (!REV) {
+        *lo = p0 + gidx * kGroup;
+        *cnt = std::min<int64_t>(kGroup, p1 - *lo);
     } else {
-#pragma unroll
-        for (int k = 0; k < kGroup; ++k) p[k] = g[k];
+        const int64_t m = (p1 - 1) / kGroup - gidx;
+        *lo = std::max<int64_t>(p0, m * kGroup);
+        *cnt = std::min<int64_t>(p1, m * kGroup + kGroup) - *lo;
     }
 }
 
-// MODE kState: chunks 0..nC-2 of every row from zero state, end states to
-// zs[(row*nC + c) * 2S]; the grid's last workgroup writes T (2S x 2S,
-// row-major) to tmat.  MODE kRun: every chunk from zs[...] (zero when zs is
-// null), outputs to y.  REV runs each row back to front.
+// One wave per chunk (see the top of the file).  MODE kState: chunks
+// 0..nC-2 of every row from zero state, end states to zs[(row*nC + c)*2S].
+// MODE kRun: every chunk from zs[...] (zero when zs is null), outputs to y.
+// REV runs each row back to front; chunk c is c-th in time, and back to
+// front the short chunk (when L does not divide N) comes first, so every
+// chunk starts at a multiple of L (16-byte aligned groups) and only the
+// first group in time can be short; every chunk after the first in time is
+// L long, so A^L is its transition.
 template <int SMAX, int MODE, bool REV>
-__global__ __launch_bounds__(kThreads) void sos_chunk_kernel(
-    const double *x, int64_t ldx, double *y, int64_t ldy, int64_t rows,
-    int64_t N, const double *__restrict__ sos, int S, int64_t L, int64_t nC,
-    double *__restrict__ zs, double *__restrict__ tmat, int vec)
+__global__ __launch_bounds__(kThreads) void sos_wave_kernel(
+    const double *x, int64_t ldx, double *y, int64_t ldy, int64_t rows, int64_t N,
+    const double *__restrict__ sos, int S, int64_t L, int64_t nC,
+    const double *__restrict__ mats, double *__restrict__ zs, int vec)
 {
+    constexpr int D = 2 * SMAX;
+    __shared__ double cm[kGroup][D];
+    __shared__ double pw[kPowers][D][D];
+    __shared__ __attribute__((aligned(16))) double tiles[kThreads / 64][kStep];
+    const int d = 2 * S;
+    for (int e = threadIdx.x; e < kGroup * D; e += kThreads) {
+        const int k = e / D, j = e % D;
+        cm[k][j] = j < d ? mats[mats_c(d) + k * d + j] : 0.0;
+    }
+    for (int e = threadIdx.x; e < kPowers * D * D; e += kThreads) {
+        const int k = e / (D * D), i = (e / D) % D, j = e % D;
+        pw[k][i][j] = (i < d && j < d) ? mats[mats_p(d) + (int64_t)k * d * d + i * d + j] : 0.0;
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    if (wave >= rows * nC) return;  // the whole wave
+    const int64_t row = wave / nC;
+    const int64_t c = wave - row * nC;
+    if (MODE == kState && c == nC - 1) return;  // its end state is never used
+    const int64_t p0 = (REV ? nC - 1 - c : c) * L;
+    const int64_t p1 = std::min<int64_t>(p0 + L, N);
+    const int64_t ngroups = REV ? (p1 - 1) / kGroup - p0 / kGroup + 1
+                                : (p1 - p0 + kGroup - 1) / kGroup;
+    const int64_t nsteps = (ngroups + 63) / 64;
+    Coefs<SMAX> cf;
+    load_coefs(cf, sos, S);
+    double zin[D];  // the wave's state at the start of the step (same in every lane)
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+        zin[i] = (MODE == kRun && zs != nullptr && i < d) ? zs[(row * nC + c) * d + i] : 0.0;
+    // x and y may be the same array (filtfilt's second pass runs in place):
+    // a step's samples are all read before any is written.
+    const double *xr = x + row * ldx;
+    double *yr = (MODE == kRun) ? y + row * ldy : nullptr;
+    const bool vb = vec != 0;
+    double *tile = tiles[threadIdx.x >> 6];
+    // A step whose 64 groups are all whole covers 8 KB of contiguous memory:
+    // 64 lines of 128 bytes, line lam = lane forward (lane 63 - lam back to
+    // front).  It moves through the wave's LDS tile: memory instruction k
+    // carries 16-byte pieces of lines 8k..8k+7 (a permutation of 1 KB, so
+    // it is coalesced), and piece j of line lam sits in tile slot
+    //   64*(lam/8) + 8*(lam%8) + (j + lam%8) % 8     (slots of 16 bytes),
+    // which makes both the instruction-ordered and the line-ordered LDS
+    // accesses at most 2-way bank conflicted.
+    const int lam = REV ? 63 - lane : lane;
+    const int own_base = 64 * (lam >> 3) + 8 * (lam & 7);  // + (j + lam%8) % 8
+    // this lane's piece in memory instruction k: line 8k + lane/8, piece pj
+    const int mline = lane >> 3;
+    const int pj = ((lane & 7) - mline) & 7;
+
+    for (int64_t st = 0; st < nsteps; ++st) {
+        // The matrices are re-read from LDS every step: an opaque zero
+        // offset keeps the compiler from holding all of them in registers.
+        int zoff = 0;
+        asm volatile("" : "+s"(zoff));
+        const double(*cmv)[D] = cm + zoff;
+        const double(*pwv)[D][D] = pw + zoff;
+        int64_t lo, cnt;
+        group_span<REV>(st * 64 + lane, ngroups, p0, p1, &lo, &cnt);
+        const bool full = cnt == kGroup;
+        const bool tiled = vb && __all(full);
+        // first sample of the step's 8 KB block when tiled
+        const int64_t blk = REV ? ((p1 - 1) / kGroup - st * 64 - 63) * kGroup : p0 + st * kStep;
+        double v[kGroup];
+        if (tiled) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const double2 t = *reinterpret_cast<const double2 *>(
+                    xr + blk + (8 * k + mline) * kGroup + 2 * pj);
+                *reinterpret_cast<double2 *>(tile + 2 * (64 * k + lane)) = t;
+            }
+            wave_sync();
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const double2 t = *reinterpret_cast<const double2 *>(
+                    tile + 2 * (own_base + ((j + lam) & 7)));
+                v[2 * j] = t.x;
+                v[2 * j + 1] = t.y;
+            }
+            wave_sync();
+        } else {
+            load_lane(v, xr, lo, cnt, vb);
+        }
+        // The lane's samples from zero state, in time order.
+        double s1[SMAX], s2[SMAX];
+#pragma unroll
+        for (int i = 0; i < SMAX; ++i) s1[i] = s2[i] = 0.0;
+        if (__all(full)) {
+            if (!REV) {
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) v[k] = df2t_step<SMAX>(v[k], s1, s2, cf, S);
+            } else {
+#pragma unroll
+                for (int k = kGroup - 1; k >= 0; --k) v[k] = df2t_step<SMAX>(v[k], s1, s2, cf, S);
+            }
+        } else {
+            if (!REV) {
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k)
+                    if (k < cnt) v[k] = df2t_step<SMAX>(v[k], s1, s2, cf, S);
+            } else {
+#pragma unroll
+                for (int k = kGroup - 1; k >= 0; --k)
+                    if (k < cnt) v[k] = df2t_step<SMAX>(v[k], s1, s2, cf, S);
+            }
+        }
+        double e[D];
+#pragma unroll
+        for (int i = 0; i < SMAX; ++i) {
+            e[2 * i] = s1[i];
+            e[2 * i + 1] = s2[i];
+        }
+        if (lane == 0) {  // the carried state enters through the first lane
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                double acc = e[i];
+#pragma unroll
+                for (int m = 0; m < D; ++m) acc = fma(pwv[0][i][m], zin[m], acc);
+                e[i] = acc;
+            }
+        }
+        wave_scan<D>(e, pwv, lane);
+        double z[D];  // the lane's start state
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            const double up = __shfl_up(e[i], 1, 64);
+            z[i] = lane == 0 ? zin[i] : up;
+        }
+        // carry: the state after the step's last group (back to front the
+        // first chunk in time is the short one, and its end state is used)
+        const int last = (int)std::min<int64_t>(63, ngroups - 1 - st * 64);
+#pragma unroll
+        for (int i = 0; i < D; ++i) zin[i] = __shfl(e[i], last, 64);
+        if (MODE == kRun) {
+            // y = y0 + C z; the chunk's very first lane starts from zero
+            // (and keeps the reference's bits, signed zeros included).
+            if (!(st == 0 && lane == 0 && c == 0)) {
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) {
+                    const int tk = REV ? (int)cnt - 1 - k : k;  // time index of v[k]
+                    if (k < cnt) {
+                        double acc = v[k];
+#pragma unroll
+                        for (int m = 0; m < D; ++m) acc = fma(cmv[tk][m], z[m], acc);
+                        v[k] = acc;
+                    }
+                }
+            }
+            if (tiled) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    *reinterpret_cast<double2 *>(tile + 2 * (own_base + ((j + lam) & 7))) =
+                        make_double2(v[2 * j], v[2 * j + 1]);
+                wave_sync();
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const double2 t = *reinterpret_cast<const double2 *>(tile + 2 * (64 * k + lane));
+                    *reinterpret_cast<double2 *>(yr + blk + (8 * k + mline) * kGroup + 2 * pj) = t;
+                }
+                wave_sync();
+            } else if (vb && full) {
+#pragma unroll
+                for (int k = 0; k < kGroup / 2; ++k)
+                    *reinterpret_cast<double2 *>(yr + lo + 2 * k) = make_double2(v[2 * k], v[2 * k + 1]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k)
+                    if (k < cnt) yr[lo + k] = v[k];
+            }
+        }
+    }
+    if (MODE == kState && lane == 0) {
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+            if (i < d) zs[(row * nC + c) * d + i] = zin[i];
+    }
+}
+
+// One lane per row, the reference loop exactly (reference order, and rows
+// too short to split).
+template <int SMAX, bool REV>
+__global__ __launch_bounds__(kThreads) void sos_row_kernel(const double *x, int64_t ldx,
+                                                           double *y, int64_t ldy, int64_t rows,
+                                                           int64_t N,
+                                                           const double *__restrict__ sos, int S,
+                                                           int vec)
+{
+    const int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (row >= rows) return;
     Coefs<SMAX> cf;
     load_coefs(cf, sos, S);
     double s1[SMAX], s2[SMAX];
 #pragma unroll
     for (int i = 0; i < SMAX; ++i) s1[i] = s2[i] = 0.0;
-
-    if (MODE == kState && blockIdx.x == gridDim.x - 1) {
-        // Columns of T = A^L: L zero-input steps from each unit state.
-        const int j = threadIdx.x;
-        if (j >= 2 * S) return;
-#pragma unroll
-        for (int i = 0; i < SMAX; ++i) {
-            s1[i] = (2 * i == j) ? 1.0 : 0.0;
-            s2[i] = (2 * i + 1 == j) ? 1.0 : 0.0;
-        }
-        for (int64_t t = 0; t < L; ++t) (void)df2t_step<SMAX>(0.0, s1, s2, cf, S);
-#pragma unroll
-        for (int i = 0; i < SMAX; ++i) {
-            if (i < S) {
-                tmat[(2 * i) * (2 * S) + j] = s1[i];
-                tmat[(2 * i + 1) * (2 * S) + j] = s2[i];
-            }
-        }
-        return;
-    }
-
-    const int64_t item = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (item >= rows * nC) return;
-    const int64_t row = item / nC;
-    const int64_t c = item - row * nC;
-    if (MODE == kState && c == nC - 1) return;  // its end state is never used
-    // Chunk c covers positions [p0, p1): c*L onwards forward; back to front
-    // the short chunk (when L does not divide N) is the first in time, so
-    // every chunk starts at a multiple of L (16-byte aligned groups) and
-    // every chunk whose end state is used is L long.
-    const int64_t p0 = (REV ? nC - 1 - c : c) * L;
-    const int64_t p1 = std::min<int64_t>(p0 + L, N);
-    if (MODE == kRun && zs != nullptr) {
-        const double *__restrict__ z = zs + item * (2 * S);
-#pragma unroll
-        for (int i = 0; i < SMAX; ++i) {
-            if (i < S) {
-                s1[i] = z[2 * i];
-                s2[i] = z[2 * i + 1];
-            }
-        }
-    }
-    // x and y may be the same array (filtfilt's second pass runs in place):
-    // a lane reads each group before it writes it and never reads it again.
     const double *xr = x + row * ldx;
-    double *yr = (MODE == kRun) ? y + row * ldy : nullptr;
-    const bool v = vec != 0;
-    double g[kGroup];
-
-    const int64_t nfull = (p1 - p0) / kGroup;
+    double *yr = y + row * ldy;
     if (!REV) {
-        int64_t t = p0;
-        const int64_t full_end = p0 + nfull * kGroup;
-        if (t < full_end) {
-            double nx[kGroup];
-            load_group(nx, xr + t, v);
-            for (; t < full_end; t += kGroup) {
+        int64_t t = 0;
+        if (vec) {
+            for (; t + kGroup <= N; t += kGroup) {
+                double v[kGroup];
+                load_lane(v, xr, t, kGroup, true);
 #pragma unroll
-                for (int k = 0; k < kGroup; ++k) g[k] = nx[k];
-                if (t + kGroup < full_end) load_group(nx, xr + t + kGroup, v);
+                for (int k = 0; k < kGroup; ++k) v[k] = df2t_step<SMAX>(v[k], s1, s2, cf, S);
 #pragma unroll
-                for (int k = 0; k < kGroup; ++k) g[k] = df2t_step<SMAX>(g[k], s1, s2, cf, S);
-                if (MODE == kRun) store_group(yr + t, g, v);
+                for (int k = 0; k < kGroup / 2; ++k)
+                    *reinterpret_cast<double2 *>(yr + t + 2 * k) = make_double2(v[2 * k], v[2 * k + 1]);
             }
         }
-        for (; t < p1; ++t) {
-            const double o = df2t_step<SMAX>(xr[t], s1, s2, cf, S);
-            if (MODE == kRun) yr[t] = o;
-        }
+        for (; t < N; ++t) yr[t] = df2t_step<SMAX>(xr[t], s1, s2, cf, S);
     } else {
-        // Descending positions: the samples above the last whole group
-        // first, then whole groups at descending addresses.
-        const int64_t p_grp_top = p0 + nfull * kGroup;  // groups cover [p0, p_grp_top)
-        for (int64_t p = p1 - 1; p >= p_grp_top; --p) {
-            const double o = df2t_step<SMAX>(xr[p], s1, s2, cf, S);
-            if (MODE == kRun) yr[p] = o;
-        }
-        if (nfull > 0) {
-            int64_t base = p_grp_top - kGroup;
-            double nx[kGroup];
-            load_group(nx, xr + base, v);
-            for (; base >= p0; base -= kGroup) {
-#pragma unroll
-                for (int k = 0; k < kGroup; ++k) g[k] = nx[k];
-                if (base - kGroup >= p0) load_group(nx, xr + base - kGroup, v);
-#pragma unroll
-                for (int k = kGroup - 1; k >= 0; --k) g[k] = df2t_step<SMAX>(g[k], s1, s2, cf, S);
-                if (MODE == kRun) store_group(yr + base, g, v);
-            }
-        }
-    }
-
-    if (MODE == kState) {
-        double *__restrict__ z = zs + item * (2 * S);
-#pragma unroll
-        for (int i = 0; i < SMAX; ++i) {
-            if (i < S) {
-                z[2 * i] = s1[i];
-                z[2 * i + 1] = s2[i];
-            }
-        }
+        for (int64_t t = N - 1; t >= 0; --t) yr[t] = df2t_step<SMAX>(xr[t], s1, s2, cf, S);
     }
 }
 
@@ -327,62 +557,77 @@ bool aligned_rows(const double *p, int64_t ld)
 
 template <int SMAX, bool REV>
 int sos_launch_t(const double *x, int64_t ldx, double *y, int64_t ldy, int64_t rows, int64_t N,
-                 const double *sos, int S, int64_t L, int64_t nC, double *zs, double *tmat,
-                 hipStream_t s)
+                 const double *sos, int S, bool row_mode, int64_t L, int64_t nC, double *mats,
+                 double *zs, hipStream_t s)
 {
     const int vec = aligned_rows(x, ldx) && aligned_rows(y, ldy);
-    const int64_t items = rows * nC;
-    const int64_t blocks = (items + kThreads - 1) / kThreads;
+    if (row_mode) {
+        hipLaunchKernelGGL((sos_row_kernel<SMAX, REV>), dim3((unsigned)((rows + kThreads - 1) / kThreads)),
+                           dim3(kThreads), 0, s, x, ldx, y, ldy, rows, N, sos, S, vec);
+        return check_launch("sos_row_kernel");
+    }
+    hipLaunchKernelGGL((sos_setup_kernel<SMAX>), dim3(1), dim3(kThreads), 0, s, sos, S,
+                       nC > 1 ? L / kGroup : (int64_t)0, mats);
+    SCIR_TRY(check_launch("sos_setup_kernel"));
+    const int64_t waves = rows * nC;
+    const unsigned blocks = (unsigned)((waves + kThreads / 64 - 1) / (kThreads / 64));
     if (nC > 1) {
-        hipLaunchKernelGGL((sos_chunk_kernel<SMAX, kState, REV>), dim3((unsigned)(blocks + 1)),
-                           dim3(kThreads), 0, s, x, ldx, y, ldy, rows, N, sos, S, L, nC, zs,
-                           tmat, vec);
-        SCIR_TRY(check_launch("sos_chunk_kernel(state)"));
+        hipLaunchKernelGGL((sos_wave_kernel<SMAX, kState, REV>), dim3(blocks), dim3(kThreads), 0, s,
+                           x, ldx, y, ldy, rows, N, sos, S, L, nC, mats, zs, vec);
+        SCIR_TRY(check_launch("sos_wave_kernel(state)"));
         hipLaunchKernelGGL((sos_scan_kernel<SMAX>), dim3((unsigned)((rows + 3) / 4)),
-                           dim3(kThreads), 0, s, zs, tmat, rows, nC, S);
+                           dim3(kThreads), 0, s, zs, mats + mats_t(2 * S), rows, nC, S);
         SCIR_TRY(check_launch("sos_scan_kernel"));
     }
-    hipLaunchKernelGGL((sos_chunk_kernel<SMAX, kRun, REV>), dim3((unsigned)blocks),
-                       dim3(kThreads), 0, s, x, ldx, y, ldy, rows, N, sos, S, L, nC,
-                       nC > 1 ? zs : nullptr, tmat, vec);
-    return check_launch("sos_chunk_kernel(run)");
+    hipLaunchKernelGGL((sos_wave_kernel<SMAX, kRun, REV>), dim3(blocks), dim3(kThreads), 0, s, x,
+                       ldx, y, ldy, rows, N, sos, S, L, nC, mats, nC > 1 ? zs : nullptr, vec);
+    return check_launch("sos_wave_kernel(run)");
 }
 
 template <bool REV>
 int sos_launch_rev(const double *x, int64_t ldx, double *y, int64_t ldy, int64_t rows, int64_t N,
-                   const double *sos, int S, int64_t L, int64_t nC, double *zs, double *tmat,
-                   hipStream_t s)
+                   const double *sos, int S, bool row_mode, int64_t L, int64_t nC, double *mats,
+                   double *zs, hipStream_t s)
 {
-#define SCIR_SOS_CASE(n) \
-    case n: return sos_launch_t<n, REV>(x, ldx, y, ldy, rows, N, sos, S, L, nC, zs, tmat, s);
+#define SCIR_SOS_CASE(n)                                                                     \
+    case n:                                                                                  \
+        return sos_launch_t<n, REV>(x, ldx, y, ldy, rows, N, sos, S, row_mode, L, nC, mats, \
+                                    zs, s);
     switch (S) {
         SCIR_SOS_CASE(1) SCIR_SOS_CASE(2) SCIR_SOS_CASE(3) SCIR_SOS_CASE(4)
         SCIR_SOS_CASE(5) SCIR_SOS_CASE(6) SCIR_SOS_CASE(7) SCIR_SOS_CASE(8)
-        default: return sos_launch_t<kMaxSections, REV>(x, ldx, y, ldy, rows, N, sos, S, L, nC,
-                                                        zs, tmat, s);
+        default:
+            return sos_launch_t<kMaxSections, REV>(x, ldx, y, ldy, rows, N, sos, S, row_mode, L,
+                                                   nC, mats, zs, s);
     }
 #undef SCIR_SOS_CASE
 }
 
 }  // namespace
 
-// Chunk length: enough chunks for ~4 waves per SIMD, at least 256 samples
-// (the chunk states stay small next to the data) and at most 4096 chunks a
-// row (the scan walks them 64 at a time).  One chunk per row in reference
-// order.
-int64_t sos_chunk_length(int64_t rows, int64_t N, int cu_count, bool reference_order)
+// Work split.  Reference order, and rows of at most one wave step, run one
+// lane per row.  Otherwise one wave per chunk: whole rows when there are
+// enough of them for ~4 waves per SIMD, else chunks of L samples (a
+// multiple of one wave step, at most 4096 chunks a row: the chunk scan
+// walks them 64 at a time).
+void sos_plan(int64_t rows, int64_t N, int cu_count, bool reference_order, bool *row_mode,
+              int64_t *L, int64_t *nC)
 {
-    if (reference_order || N <= 256) return std::max<int64_t>(N, 1);
-    const int64_t target_lanes = (int64_t)cu_count * 4 * 4 * 64;
-    auto up16 = [](int64_t v) { return (v + kGroup - 1) / kGroup * kGroup; };
-    int64_t L = up16((rows * N + target_lanes - 1) / target_lanes);
-    L = std::max<int64_t>(L, 256);
-    L = std::max<int64_t>(L, up16((N + 4095) / 4096));
-    return std::min<int64_t>(L, N);
+    *row_mode = reference_order || N <= kStep;
+    *L = N;
+    *nC = 1;
+    if (*row_mode) return;
+    const int64_t target_waves = (int64_t)cu_count * 16;
+    if (rows >= target_waves) return;
+    auto up = [](int64_t v) { return (v + kStep - 1) / kStep * kStep; };
+    int64_t l = up((rows * N + target_waves - 1) / target_waves);
+    l = std::max<int64_t>(l, up((N + 4095) / 4096));
+    if (l >= N) return;
+    *L = l;
+    *nC = (N + l - 1) / l;
 }
 
-// Validates arguments, sizes the chunk-state scratch and runs one direction
-// (caller holds st->mu).
+// Runs one direction (caller holds st->mu).
 int sos_run(const double *x, int64_t ldx, double *y, int64_t ldy, int64_t rows, int64_t N,
             const double *sos, int S, bool reference_order, bool rev, hipStream_t s,
             DeviceState *st)
@@ -393,21 +638,24 @@ int sos_run(const double *x, int64_t ldx, double *y, int64_t ldy, int64_t rows, 
                                           N * sizeof(double), rows, hipMemcpyDeviceToDevice, s));
         return SCIR_HIP_OK;
     }
-    const int64_t L = sos_chunk_length(rows, N, st->cu_count, reference_order);
-    const int64_t nC = (N + L - 1) / L;
-    if (rows * nC / kThreads + 2 > 0x7fffffffLL)
+    bool row_mode = false;
+    int64_t L = 0, nC = 0;
+    sos_plan(rows, N, st->cu_count, reference_order, &row_mode, &L, &nC);
+    if ((row_mode ? rows / kThreads : rows * nC / (kThreads / 64)) + 2 > 0x7fffffffLL)
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "too many rows");
-    double *zs = nullptr, *tmat = nullptr;
-    if (nC > 1) {
+    double *mats = nullptr, *zs = nullptr;
+    if (!row_mode) {
         StreamScratch &sc = scratch_for(st, s);
-        const size_t zbytes = (size_t)(rows * nC * 2 * S) * sizeof(double);
-        const size_t tbytes = (size_t)(4 * S * S) * sizeof(double);
-        SCIR_TRY(ensure_buffer(sc.sos_states, zbytes + tbytes + 256, s));
-        zs = static_cast<double *>(sc.sos_states.ptr);
-        tmat = zs + (zbytes / sizeof(double) + 31) / 32 * 32;
+        const int d = 2 * S;
+        const int64_t mdoubles = (mats_size(d) + 31) / 32 * 32;
+        const int64_t zdoubles = nC > 1 ? rows * nC * d : 0;
+        SCIR_TRY(ensure_buffer(sc.sos_states, (size_t)(mdoubles + zdoubles) * sizeof(double), s));
+        mats = static_cast<double *>(sc.sos_states.ptr);
+        zs = mats + mdoubles;
     }
-    if (rev) return sos_launch_rev<true>(x, ldx, y, ldy, rows, N, sos, S, L, nC, zs, tmat, s);
-    return sos_launch_rev<false>(x, ldx, y, ldy, rows, N, sos, S, L, nC, zs, tmat, s);
+    if (rev)
+        return sos_launch_rev<true>(x, ldx, y, ldy, rows, N, sos, S, row_mode, L, nC, mats, zs, s);
+    return sos_launch_rev<false>(x, ldx, y, ldy, rows, N, sos, S, row_mode, L, nC, mats, zs, s);
 }
 
 int sos_args(const double *x, int64_t ldx, int64_t rows, int64_t N, const double *sos, int32_t S,

@@ -1,10 +1,14 @@
 """sosfilt / filtfilt on the GPU against the oracle (the reference loop).
 
-reference_order: every output equal to the reference loop bit for bit.
-Chunked (default): the first 256 outputs of each row bit for bit (the first
-chunk starts from zero state like the reference), every output within
-1e-12 of the row's largest |y| (the chunk joins carry the rounding of the
-state transition; measured errors are a few 1e-16)."""
+reference_order (and rows of at most 1024 samples): every output equal to
+the reference loop bit for bit.  Parallel in time (default): the first 16
+outputs of each row bit for bit (they run from the zero start state like the
+reference), and every output within 1e-12 of the row's largest |y| of the
+reference loop (the joins carry the rounding of the state transitions;
+measured ~1e-15).  Ill-conditioned cascades (high-order Chebyshev) make the
+reference loop itself err by more than that: then both are compared with an
+extended-precision run, and the GPU's error may be at most 16 times the
+reference loop's (plus 1e-14 of the row's largest |y|)."""
 import pathlib
 
 import numpy as np
@@ -30,14 +34,20 @@ def _sos(S, seed=0):
     return np.ascontiguousarray(base[seed % 3][:S])
 
 
-def _check(y, ref, exact_prefix):
+def _check(y, ref, exact_prefix, sos=None, x=None, zero_phase=False):
     assert y.shape == ref.shape
     if exact_prefix:
-        k = min(256, ref.shape[1])
+        k = min(16, ref.shape[1])
         assert np.array_equal(y[:, :k], ref[:, :k])
     scale = np.max(np.abs(ref), axis=1, keepdims=True) + 1e-300
     err = np.max(np.abs(y - ref) / scale)
-    assert err <= REL_TOL, err
+    if err <= REL_TOL:
+        return
+    assert sos is not None, err
+    exact = orc.sosfilt_longdouble(sos, x, zero_phase).astype(np.float64)
+    err_gpu = np.max(np.abs(y - exact) / scale)
+    err_ref = np.max(np.abs(ref - exact) / scale)
+    assert err_gpu <= 16 * err_ref + 1e-14, (err_gpu, err_ref)
 
 
 @pytest.mark.parametrize("S", [1, 2, 3, 4, 5, 6, 7, 8, 9, 16])
@@ -60,8 +70,9 @@ def test_chunked_close(S, shape):
     rng = np.random.default_rng(S * 11 + shape[1])
     x = rng.standard_normal(shape)
     sos = _sos(S, S + 1)
-    _check(signal.sosfilt_batched(sos, x, Device.Hip), orc.sosfilt_f64(sos, x), True)
-    _check(signal.filtfilt_batched(sos, x, Device.Hip), orc.filtfilt_f64(sos, x), False)
+    _check(signal.sosfilt_batched(sos, x, Device.Hip), orc.sosfilt_f64(sos, x), True, sos, x)
+    _check(signal.filtfilt_batched(sos, x, Device.Hip), orc.filtfilt_f64(sos, x), False, sos, x,
+           zero_phase=True)
 
 
 @pytest.mark.parametrize("design", ["low_pass", "high_pass", "band_pass", "band_stop"])
@@ -73,8 +84,8 @@ def test_reference_iir_fixtures(design):
     for ref_order in (False, True):
         y = signal.sosfilt_batched(sos, x[None, :], Device.Hip, reference_order=ref_order)[0]
         assert np.max(np.abs(y - want)) <= 1e-12
-    assert np.array_equal(signal.sosfilt(sos, x, Device.Hip)[:256],
-                          orc.sosfilt_f64(sos, x)[:256])
+    assert np.array_equal(signal.sosfilt(sos, x, Device.Hip)[:16],
+                          orc.sosfilt_f64(sos, x)[:16])
 
 
 def test_signal_fixtures_and_edges():

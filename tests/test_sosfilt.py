@@ -104,3 +104,21 @@ def test_mirror_contract():
         signal.sosfilt_batched(np.zeros((2, 5)), np.zeros((1, 4)))
     with pytest.raises(ShapeMismatch):
         signal.sosfilt_batched(np.zeros((17, 6)), np.zeros((1, 4)))
+
+
+def test_longdouble_oracle():
+    # the extended-precision cascade: the resonator closed form, and the
+    # float64 loop within its rounding for a well-conditioned filter
+    r, th = 0.97, 0.3
+    sos = np.array([[1.0, 0.0, 0.0, 1.0, -2 * r * np.cos(th), r * r]])
+    n = np.arange(400)
+    imp = np.zeros((1, 400))
+    imp[0, 0] = 1.0
+    want = r ** n * np.sin((n + 1) * th) / np.sin(th)
+    assert np.max(np.abs(orc.sosfilt_longdouble(sos, imp)[0].astype(np.float64) - want)) <= 1e-13
+    sos = ss.butter(6, 0.15, output="sos")
+    x = np.random.default_rng(2).standard_normal((2, 500))
+    assert np.max(np.abs(orc.sosfilt_longdouble(sos, x).astype(np.float64)
+                         - orc.sosfilt_f64(sos, x))) <= 1e-14
+    assert np.max(np.abs(orc.sosfilt_longdouble(sos, x, True).astype(np.float64)
+                         - orc.filtfilt_f64(sos, x))) <= 1e-14
