@@ -11,6 +11,7 @@ in HBM.  ``value`` = output samples / second over the K timed steps (device
 time, HIP events on the launch stream).  Beside it the line carries, measured
 after the timed region on the same inputs:
   compare.fp32_direct / compare.fp16  the other arithmetic paths, Gsamples/s
+  / compare.fft32                     (fft32: FP32 FFT overlap-save, normwise error)
                                       and rel-L2 vs the FP64 oracle
   bandwidth_workload                  x [4096, 65536], 33-tap low-pass, FP32
                                       direct: GB/s against the 8 TB/s HBM peak
@@ -187,7 +188,8 @@ def roofline_for(wl, name, prec, avg_launch_s):
                     mfma_busy_frac=load_profile(f"fir_mfma_{prec}_{name}_util", "mfma_busy_frac"),
                     sclk_ghz=load_profile(f"fir_mfma_{prec}_{name}_util", "sclk_ghz"))
     flops = 2.0 * wl["K"] * samples
-    if flops / (8.0 * samples) < 78.6 / 8.0:  # below the FP32-vector ridge: HBM-bound
+    if prec == "fft32" or flops / (8.0 * samples) < 78.6 / 8.0:
+        # HBM-bound (the FFT path's work per sample is O(log K))
         alg = 8.0 * samples
         ach = alg / avg_launch_s / 1e9
         return dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
@@ -223,8 +225,8 @@ def run_dense(args):
                roofline=roofline_for(wl, name, prec, float(np.mean(per)) * 1e-3))
     if not args.quick:
         comp = {}
-        for other in ("f32", "fp16", "bf16x3"):
-            if other == prec:
+        for other in ("f32", "fp16", "bf16x3", "fft32"):
+            if other == prec or (other == "fft32" and K > 8192):
                 continue
             ms_o, _, _ = timed(lambda: device.fir(x, b, other, out=y), max(3, args.steps // 4),
                                1, stream)

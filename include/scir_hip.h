@@ -72,8 +72,13 @@ typedef enum scir_hip_precision {
      * rounded to fp16), FP32 accumulation.  Reported, not accuracy-bound. */
     SCIR_HIP_PREC_FP16 = 3,
     /* Fastest of F32 / BF16X3 for the shape (autotuned table when loaded,
-     * else built-in cut-over); never FP16. */
-    SCIR_HIP_PREC_AUTO = 4
+     * else built-in cut-over); never FP16 or FFT32. */
+    SCIR_HIP_PREC_AUTO = 4,
+    /* FP32 FFT overlap-save (K <= 8192): cost O(log K) per output instead of
+     * O(K); the error is normwise (about log2(NF) * 2^-24 * |x|_2 |b|_1 over
+     * each NF = 4096, 8192 or 16384 sample segment), not elementwise.  x and
+     * y must not be the same array. */
+    SCIR_HIP_PREC_FFT32 = 5
 } scir_hip_precision;
 
 /* ---- runtime ------------------------------------------------------------ */

@@ -27,9 +27,10 @@ PREC_F32_EXACT = 1
 PREC_BF16X3 = 2
 PREC_FP16 = 3
 PREC_AUTO = 4
+PREC_FFT32 = 5
 
 PRECISIONS = {"f32": PREC_F32, "f32_exact": PREC_F32_EXACT, "bf16x3": PREC_BF16X3,
-              "fp16": PREC_FP16, "auto": PREC_AUTO}
+              "fp16": PREC_FP16, "auto": PREC_AUTO, "fft32": PREC_FFT32}
 
 # Every function of include/scir_hip.h: name -> (restype, argtypes).
 _c = ctypes

@@ -48,6 +48,7 @@ struct DeviceBuffer {
 struct StreamScratch {
     DeviceBuffer partials;    // per-workgroup dL/db partial sums
     DeviceBuffer sos_states;  // per-chunk IIR states and the chunk transition
+    DeviceBuffer fft_spectrum;  // the filter's spectrum for the FFT path
 };
 
 struct DeviceState {
@@ -89,6 +90,8 @@ int check_launch(const char *what);
 int fir_direct_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
                    float *y, int64_t ldy, bool exact, bool rev, hipStream_t stream,
                    DeviceState *st);
+int fir_fft_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
+                float *y, int64_t ldy, bool rev, hipStream_t stream, DeviceState *st);
 int fir_mfma_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
                  float *y, int64_t ldy, int precision, bool rev, hipStream_t stream,
                  DeviceState *st);

@@ -371,6 +371,8 @@ int fir_dense_dispatch(const float *x, int64_t ldx, const float *b, int32_t K, i
     case SCIR_HIP_PREC_BF16X3:
     case SCIR_HIP_PREC_FP16:
         return fir_mfma_dev(x, ldx, b, K, B, N, y, ldy, precision, rev, stream, st);
+    case SCIR_HIP_PREC_FFT32:
+        return fir_fft_dev(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
     case SCIR_HIP_PREC_AUTO: {
         const TuneChoice tc = tune_lookup(K, N);
         if (K >= tc.auto_mfma_min_k)
@@ -407,7 +409,7 @@ int scir_hip_fir_f32_host(const float *x, int64_t ldx, const float *b, int32_t K
     clear_error();
     SCIR_TRY(check_dense_args(x, ldx, b, K, B, N, y, ldy));
     if (B == 0 || N == 0) return SCIR_HIP_OK;
-    if (precision < SCIR_HIP_PREC_F32 || precision > SCIR_HIP_PREC_AUTO)
+    if (precision < SCIR_HIP_PREC_F32 || precision > SCIR_HIP_PREC_FFT32)
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "unknown precision %d", precision);
     DeviceState *st = nullptr;
     SCIR_TRY(current_device_state(&st));

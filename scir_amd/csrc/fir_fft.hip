@@ -1,0 +1,367 @@
+// FP32 FFT overlap-save FIR for long filters (SCIR_HIP_PREC_FFT32).
+//
+// The direct and Toeplitz-MFMA kernels cost O(K) per output; a segment of
+// NF samples convolved through the FFT costs O(log NF) per output.  Each
+// workgroup filters two segments of NF - K + 1 outputs at once: their inputs
+// (the K - 1 samples before each plus the segment) are the real and
+// imaginary parts of one complex NF-point signal z; since the filter is
+// real, IFFT(FFT(z) * H) carries the first segment's circular convolution in
+// its real part and the second's in its imaginary part, and entries K-1 ..
+// NF-1 of a circular convolution are the linear convolution's outputs.
+// H = FFT(b zero-padded to NF) / NF comes from fir_fft_spectrum_kernel,
+// launched first on the same stream.
+//
+// The transforms run in LDS: radix-16/32 Stockham passes (natural order in
+// and out), each thread loading R points with stride NF/R, multiplying by
+// the pass twiddles, taking an R-point DFT in registers (radix-2 stages
+// with compile-time twiddles) and storing with stride NS.  LDS indices are
+// padded by one complex every 32 (i + i/32), which keeps every pass's loads
+// and stores free of bank conflicts.  Pass twiddles are products of two
+// small tables (64 fine and NF/64 coarse angles) computed on the host in
+// double and rounded once.
+//
+// Error: the FFT's rounding is normwise, about log2(NF) * 2^-24 * |x|_2 |b|_1
+// per segment, not relative to each output; the direct and MFMA kernels keep
+// the elementwise (K+1) 2^-24 sum|b||x| bound.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace scir {
+namespace {
+
+// ---- compile-time trigonometry for the in-register DFTs ----------------------
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+constexpr double ct_sin_taylor(double x)
+{
+    double term = x, sum = x;
+    for (int n = 1; n < 30; ++n) {
+        term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+// cos / sin of 2*pi*m/R
+constexpr double ct_cos_frac(int m, int R)
+{
+    double a = 2.0 * kPi * (double)m / (double)R;
+    if (a > kPi) a -= 2.0 * kPi;
+    return ct_sin_taylor(a + kPi / 2.0 > kPi ? a + kPi / 2.0 - 2.0 * kPi : a + kPi / 2.0);
+}
+constexpr double ct_sin_frac(int m, int R)
+{
+    double a = 2.0 * kPi * (double)m / (double)R;
+    if (a > kPi) a -= 2.0 * kPi;
+    return ct_sin_taylor(a);
+}
+
+template <int R>
+struct SmallTw {
+    float c[R], s[R];
+    constexpr SmallTw() : c(), s()
+    {
+        for (int m = 0; m < R; ++m) {
+            c[m] = (float)ct_cos_frac(m, R);
+            s[m] = (float)ct_sin_frac(m, R);
+        }
+    }
+};
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// v <- DFT_R(v) (forward: exp(-2 pi i f n / R); INV: exp(+...)), natural
+// order in and out.  Radix-2 decimation in frequency, then the bit-reversal
+// permutation (a renaming of registers).
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2 (&v)[R])
+{
+    constexpr SmallTw<R> tw{};
+#pragma unroll
+    for (int len = R; len >= 2; len >>= 1) {
+        const int half = len / 2;
+#pragma unroll
+        for (int s0 = 0; s0 < R; s0 += len) {
+#pragma unroll
+            for (int j = 0; j < half; ++j) {
+                const float2 a = v[s0 + j], b = v[s0 + j + half];
+                v[s0 + j] = cadd(a, b);
+                const float2 t = csub(a, b);
+                const int m = j * (R / len);  // W_len^j = W_R^m
+                if (m == 0) {
+                    v[s0 + j + half] = t;
+                } else if (4 * m == R) {  // -i (forward) or +i (inverse)
+                    v[s0 + j + half] = INV ? make_float2(-t.y, t.x) : make_float2(t.y, -t.x);
+                } else {
+                    const float2 w = make_float2(tw.c[m], INV ? tw.s[m] : -tw.s[m]);
+                    v[s0 + j + half] = cmul(t, w);
+                }
+            }
+        }
+    }
+    float2 o[R];
+#pragma unroll
+    for (int f = 0; f < R; ++f) {
+        int r = 0;
+#pragma unroll
+        for (int b = 1, q = f; b < R; b <<= 1, q >>= 1) r = (r << 1) | (q & 1);
+        o[f] = v[r];
+    }
+#pragma unroll
+    for (int f = 0; f < R; ++f) v[f] = o[f];
+}
+
+__device__ __forceinline__ int pad(int i) { return i + (i >> 5); }
+
+// exp(-+ 2 pi i m / NF), m < NF, from two small tables of (cos, sin):
+// tq[i] at angle 2*pi*i/NF (i < 64) and tq[64 + a] at 2*pi*64a/NF
+// (a < NF/64); their product is within about 1.5 ulp.
+constexpr int kFine = 64;
+template <int NF>
+constexpr int tw_entries() { return kFine + NF / kFine; }
+
+template <int NF, bool INV>
+__device__ __forceinline__ float2 twiddle(const float2 *tq, int m)
+{
+    const float2 p = cmul(tq[m & (kFine - 1)], tq[kFine + m / kFine]);
+    return make_float2(p.x, INV ? p.y : -p.y);
+}
+
+// One Stockham pass: radix R, NS = product of the earlier radices.
+// MULH multiplies the loaded points by the spectrum H first (the pointwise
+// product, fused into the inverse transform's first pass).
+template <int NF, int T, int R, int NS, bool INV, bool MULH>
+__device__ __forceinline__ void fft_pass(float2 *d, const float2 *tq,
+                                         const float2 *__restrict__ H, int tid)
+{
+    constexpr int B = NF / R;
+    constexpr int PER = (B + T - 1) / T;
+    float2 v[PER][R];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int j = tid + p * T;
+        if (B % T == 0 || j < B) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                v[p][r] = d[pad(j + r * B)];
+                if (MULH) v[p][r] = cmul(v[p][r], H[j + r * B]);
+            }
+            if (NS > 1) {
+                const int k = j % NS;
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    v[p][r] = cmul(v[p][r], twiddle<NF, INV>(tq, k * r * (NF / (NS * R))));
+            }
+            dft<R, INV>(v[p]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int j = tid + p * T;
+        if (B % T == 0 || j < B) {
+            const int k = j % NS;
+            const int base = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) d[pad(base + r * NS)] = v[p][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int LOGN> struct Plan;
+template <> struct Plan<12> { static constexpr int T = 256, R0 = 16, R1 = 16, R2 = 16; };
+template <> struct Plan<13> { static constexpr int T = 256, R0 = 16, R1 = 16, R2 = 32; };
+template <> struct Plan<14> { static constexpr int T = 512, R0 = 16, R1 = 32, R2 = 32; };
+
+template <int LOGN, bool INV, bool MULH>
+__device__ __forceinline__ void fft(float2 *d, const float2 *tq, const float2 *__restrict__ H,
+                                    int tid)
+{
+    using P = Plan<LOGN>;
+    constexpr int NF = 1 << LOGN;
+    fft_pass<NF, P::T, P::R0, 1, INV, MULH>(d, tq, H, tid);
+    fft_pass<NF, P::T, P::R1, P::R0, INV, false>(d, tq, H, tid);
+    fft_pass<NF, P::T, P::R2, P::R0 * P::R1, INV, false>(d, tq, H, tid);
+}
+
+template <int LOGN>
+constexpr size_t lds_bytes()
+{
+    constexpr int NF = 1 << LOGN;
+    return (size_t)(NF + NF / 32 + tw_entries<NF>()) * sizeof(float2);
+}
+
+// H[i] = FFT(b zero-padded)[i] / NF, one workgroup.
+template <int LOGN>
+__global__ __launch_bounds__(Plan<LOGN>::T) void fir_fft_spectrum_kernel(
+    const float *__restrict__ b, int K, const float2 *__restrict__ tq_g, float2 *__restrict__ H)
+{
+    constexpr int NF = 1 << LOGN, T = Plan<LOGN>::T;
+    extern __shared__ float2 lds_f2[];
+    float2 *d = lds_f2;
+    float2 *tq = lds_f2 + NF + NF / 32;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < tw_entries<NF>(); i += T) tq[i] = tq_g[i];
+    for (int i = tid; i < NF; i += T) d[pad(i)] = make_float2(i < K ? b[i] : 0.0f, 0.0f);
+    __syncthreads();
+    fft<LOGN, false, false>(d, tq, nullptr, tid);
+    const float inv = 1.0f / (float)NF;  // exact: NF is a power of two
+    for (int i = tid; i < NF; i += T) {
+        const float2 v = d[pad(i)];
+        H[i] = make_float2(v.x * inv, v.y * inv);
+    }
+}
+
+// Segment s of the (row, segment) list: row s / spr, outputs [o, o + V),
+// V = NF - K + 1, o = (s % spr) * V.  REV reads and writes every row back to
+// front (the anti-causal correlation of the dL/dx adjoint).
+template <int LOGN, bool REV>
+__global__ __launch_bounds__(Plan<LOGN>::T) void fir_fft_kernel(
+    const float *__restrict__ x, int64_t ldx, float *__restrict__ y, int64_t ldy, int64_t N,
+    int K, const float2 *__restrict__ H, const float2 *__restrict__ tq_g, int64_t spr,
+    int64_t nseg)
+{
+    constexpr int NF = 1 << LOGN, T = Plan<LOGN>::T;
+    extern __shared__ float2 lds_f2[];
+    float2 *d = lds_f2;
+    float2 *tq = lds_f2 + NF + NF / 32;
+    const int tid = threadIdx.x;
+    const int V = NF - K + 1;
+    const int64_t s0 = 2 * (int64_t)blockIdx.x, s1 = s0 + 1;
+    const bool has1 = s1 < nseg;
+    const int64_t ra = s0 / spr, oa = (s0 - ra * spr) * V;
+    const int64_t rb = has1 ? s1 / spr : ra, ob = has1 ? (s1 - rb * spr) * V : 0;
+    const float *xa = x + ra * ldx, *xb = x + rb * ldx;
+    for (int i = tid; i < tw_entries<NF>(); i += T) tq[i] = tq_g[i];
+    for (int i = tid; i < NF; i += T) {
+        const int64_t ta = oa - (K - 1) + i, tb = ob - (K - 1) + i;
+        float va = 0.0f, vb = 0.0f;
+        if (ta >= 0 && ta < N) va = xa[REV ? N - 1 - ta : ta];
+        if (has1 && tb >= 0 && tb < N) vb = xb[REV ? N - 1 - tb : tb];
+        d[pad(i)] = make_float2(va, vb);
+    }
+    __syncthreads();
+    fft<LOGN, false, false>(d, tq, nullptr, tid);
+    fft<LOGN, true, true>(d, tq, H, tid);
+    float *ya = y + ra * ldy, *yb = y + rb * ldy;
+    for (int v = tid; v < V; v += T) {
+        const float2 o = d[pad(K - 1 + v)];
+        const int64_t ta = oa + v, tb = ob + v;
+        if (ta < N) ya[REV ? N - 1 - ta : ta] = o.x;
+        if (has1 && tb < N) yb[REV ? N - 1 - tb : tb] = o.y;
+    }
+}
+
+// Quarter-wave tables, one per size, built on first use per device.
+struct TwTables {
+    DeviceBuffer tab[3];
+};
+TwTables g_tw[64];
+
+int twiddle_table(DeviceState *st, int logn, hipStream_t s, const float2 **out)
+{
+    const int NF = 1 << logn;
+    TwTables &t = g_tw[st->device & 63];
+    DeviceBuffer &buf = t.tab[logn - 12];
+    if (!buf.ptr) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (s != nullptr && hipStreamIsCapturing(s, &cs) == hipSuccess &&
+            cs != hipStreamCaptureStatusNone)
+            return fail(SCIR_HIP_ERR_CAPTURE,
+                        "the FFT twiddle table is built on first use; run once outside capture");
+        std::vector<float2> h(kFine + NF / kFine);
+        for (int i = 0; i < kFine; ++i) {
+            const double a = 2.0 * kPi * (double)i / (double)NF;
+            h[i] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        for (int i = 0; i < NF / kFine; ++i) {
+            const double a = 2.0 * kPi * (double)(i * kFine) / (double)NF;
+            h[kFine + i] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        SCIR_HIP_TRY(hipMalloc(&buf.ptr, h.size() * sizeof(float2)));
+        buf.bytes = h.size() * sizeof(float2);
+        SCIR_HIP_TRY(hipMemcpy(buf.ptr, h.data(), buf.bytes, hipMemcpyHostToDevice));
+    }
+    *out = static_cast<const float2 *>(buf.ptr);
+    return SCIR_HIP_OK;
+}
+
+template <int LOGN>
+int launch_fft(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N, float *y,
+               int64_t ldy, bool rev, hipStream_t s, DeviceState *st)
+{
+    constexpr int NF = 1 << LOGN, T = Plan<LOGN>::T;
+    const float2 *tq = nullptr;
+    SCIR_TRY(twiddle_table(st, LOGN, s, &tq));
+    StreamScratch &sc = scratch_for(st, s);
+    SCIR_TRY(ensure_buffer(sc.fft_spectrum, (size_t)NF * sizeof(float2), s));
+    float2 *H = static_cast<float2 *>(sc.fft_spectrum.ptr);
+    constexpr size_t lds = lds_bytes<LOGN>();
+    static bool attr_set = false;  // guarded by st->mu
+    if (!attr_set) {
+        SCIR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&fir_fft_kernel<LOGN, false>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        SCIR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&fir_fft_kernel<LOGN, true>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        SCIR_HIP_TRY(hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&fir_fft_spectrum_kernel<LOGN>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((fir_fft_spectrum_kernel<LOGN>), dim3(1), dim3(T), lds, s, b, K, tq, H);
+    SCIR_TRY(check_launch("fir_fft_spectrum_kernel"));
+    const int64_t V = NF - K + 1;
+    const int64_t spr = (N + V - 1) / V;
+    const int64_t nseg = B * spr;
+    const int64_t blocks = (nseg + 1) / 2;
+    if (blocks > 0x7fffffff) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "batch too large");
+    if (rev)
+        hipLaunchKernelGGL((fir_fft_kernel<LOGN, true>), dim3((unsigned)blocks), dim3(T), lds, s, x,
+                           ldx, y, ldy, N, K, H, tq, spr, nseg);
+    else
+        hipLaunchKernelGGL((fir_fft_kernel<LOGN, false>), dim3((unsigned)blocks), dim3(T), lds, s,
+                           x, ldx, y, ldy, N, K, H, tq, spr, nseg);
+    return check_launch("fir_fft_kernel");
+}
+
+}  // namespace
+
+// Largest K the FFT path takes: the largest transform (2^14) keeps at least
+// half of each segment as outputs.
+constexpr int kFftMaxTaps = 8192;
+
+// FFT size for K taps: the smallest of 2^12, 2^13, 2^14 with NF >= 4K
+// (outputs per segment >= 3/4 NF), else 2^14.
+int fft_log_size(int K)
+{
+    if (4 * K <= 4096) return 12;
+    if (4 * K <= 8192) return 13;
+    return 14;
+}
+
+int fir_fft_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N, float *y,
+                int64_t ldy, bool rev, hipStream_t stream, DeviceState *st)
+{
+    if (K == 0)  // no taps: zeros, as every path gives
+        return fir_direct_dev(x, ldx, b, K, B, N, y, ldy, false, rev, stream, st);
+    if (K < 0 || K > kFftMaxTaps)
+        return fail(SCIR_HIP_ERR_UNSUPPORTED, "the FFT path takes 1 <= K <= %d (K=%d)", kFftMaxTaps,
+                    K);
+    if (x == y)
+        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "the FFT path does not filter in place");
+    switch (fft_log_size(K)) {
+    case 12: return launch_fft<12>(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
+    case 13: return launch_fft<13>(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
+    default: return launch_fft<14>(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
+    }
+}
+
+}  // namespace scir

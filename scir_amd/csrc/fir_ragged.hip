@@ -125,6 +125,7 @@ int ragged_precision(int32_t p, bool *exact)
     case SCIR_HIP_PREC_F32_EXACT: *exact = true; return SCIR_HIP_OK;
     case SCIR_HIP_PREC_BF16X3:
     case SCIR_HIP_PREC_FP16:
+    case SCIR_HIP_PREC_FFT32:
         return fail(SCIR_HIP_ERR_UNSUPPORTED,
                     "ragged batches run the FP32 direct form (f32 / f32_exact / auto)");
     default: return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "unknown precision %d", p);
