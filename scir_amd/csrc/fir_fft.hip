@@ -120,12 +120,15 @@ __device__ __forceinline__ void dft(float2 (&v)[R])
 
 __device__ __forceinline__ int pad(int i) { return i + (i >> 5); }
 
-// exp(-+ 2 pi i m / NF), m < NF, from two small tables of (cos, sin):
-// tq[i] at angle 2*pi*i/NF (i < 64) and tq[64 + a] at 2*pi*64a/NF
-// (a < NF/64); their product is within about 1.5 ulp.
+// Twiddle tables (cos, sin of positive angles; the forward transform uses
+// the conjugates), in this order:
+//   tq[i], i < 64:        2*pi*i/NF        (fine)
+//   tq[64 + a], a < NF/64: 2*pi*64a/NF      (coarse; fine * coarse gives any
+//                                            angle 2*pi*m/NF within ~1.5 ulp)
+//   tq[64 + NF/64 + m]:   2*pi*m/(R0*R1)   (the second pass's angles, direct)
 constexpr int kFine = 64;
 template <int NF>
-constexpr int tw_entries() { return kFine + NF / kFine; }
+constexpr int tw_fine_coarse() { return kFine + NF / kFine; }
 
 template <int NF, bool INV>
 __device__ __forceinline__ float2 twiddle(const float2 *tq, int m)
@@ -137,7 +140,7 @@ __device__ __forceinline__ float2 twiddle(const float2 *tq, int m)
 // One Stockham pass: radix R, NS = product of the earlier radices.
 // MULH multiplies the loaded points by the spectrum H first (the pointwise
 // product, fused into the inverse transform's first pass).
-template <int NF, int T, int R, int NS, bool INV, bool MULH>
+template <int NF, int T, int R, int NS, bool INV, bool MULH, bool DIRECT = false>
 __device__ __forceinline__ void fft_pass(float2 *d, const float2 *tq,
                                          const float2 *__restrict__ H, int tid)
 {
@@ -156,8 +159,16 @@ __device__ __forceinline__ void fft_pass(float2 *d, const float2 *tq,
             if (NS > 1) {
                 const int k = j % NS;
 #pragma unroll
-                for (int r = 1; r < R; ++r)
-                    v[p][r] = cmul(v[p][r], twiddle<NF, INV>(tq, k * r * (NF / (NS * R))));
+                for (int r = 1; r < R; ++r) {
+                    float2 w;
+                    if (DIRECT) {  // tq points at this pass's table of NS*R angles
+                        const float2 t = tq[k * r];
+                        w = make_float2(t.x, INV ? t.y : -t.y);
+                    } else {
+                        w = twiddle<NF, INV>(tq, k * r * (NF / (NS * R)));
+                    }
+                    v[p][r] = cmul(v[p][r], w);
+                }
             }
             dft<R, INV>(v[p]);
         }
@@ -188,15 +199,21 @@ __device__ __forceinline__ void fft(float2 *d, const float2 *tq, const float2 *_
     using P = Plan<LOGN>;
     constexpr int NF = 1 << LOGN;
     fft_pass<NF, P::T, P::R0, 1, INV, MULH>(d, tq, H, tid);
-    fft_pass<NF, P::T, P::R1, P::R0, INV, false>(d, tq, H, tid);
+    fft_pass<NF, P::T, P::R1, P::R0, INV, false, true>(d, tq + tw_fine_coarse<NF>(), H, tid);
     fft_pass<NF, P::T, P::R2, P::R0 * P::R1, INV, false>(d, tq, H, tid);
+}
+
+template <int LOGN>
+constexpr int tw_entries()
+{
+    return tw_fine_coarse<1 << LOGN>() + Plan<LOGN>::R0 * Plan<LOGN>::R1;
 }
 
 template <int LOGN>
 constexpr size_t lds_bytes()
 {
     constexpr int NF = 1 << LOGN;
-    return (size_t)(NF + NF / 32 + tw_entries<NF>()) * sizeof(float2);
+    return (size_t)(NF + NF / 32 + tw_entries<LOGN>()) * sizeof(float2);
 }
 
 // H[i] = FFT(b zero-padded)[i] / NF, one workgroup.
@@ -209,7 +226,7 @@ __global__ __launch_bounds__(Plan<LOGN>::T) void fir_fft_spectrum_kernel(
     float2 *d = lds_f2;
     float2 *tq = lds_f2 + NF + NF / 32;
     const int tid = threadIdx.x;
-    for (int i = tid; i < tw_entries<NF>(); i += T) tq[i] = tq_g[i];
+    for (int i = tid; i < tw_entries<LOGN>(); i += T) tq[i] = tq_g[i];
     for (int i = tid; i < NF; i += T) d[pad(i)] = make_float2(i < K ? b[i] : 0.0f, 0.0f);
     __syncthreads();
     fft<LOGN, false, false>(d, tq, nullptr, tid);
@@ -240,23 +257,44 @@ __global__ __launch_bounds__(Plan<LOGN>::T) void fir_fft_kernel(
     const int64_t ra = s0 / spr, oa = (s0 - ra * spr) * V;
     const int64_t rb = has1 ? s1 / spr : ra, ob = has1 ? (s1 - rb * spr) * V : 0;
     const float *xa = x + ra * ldx, *xb = x + rb * ldx;
-    for (int i = tid; i < tw_entries<NF>(); i += T) tq[i] = tq_g[i];
-    for (int i = tid; i < NF; i += T) {
-        const int64_t ta = oa - (K - 1) + i, tb = ob - (K - 1) + i;
-        float va = 0.0f, vb = 0.0f;
-        if (ta >= 0 && ta < N) va = xa[REV ? N - 1 - ta : ta];
-        if (has1 && tb >= 0 && tb < N) vb = xb[REV ? N - 1 - tb : tb];
-        d[pad(i)] = make_float2(va, vb);
+    for (int i = tid; i < tw_entries<LOGN>(); i += T) tq[i] = tq_g[i];
+    const int64_t wa = oa - (K - 1), wb = ob - (K - 1);  // first input of each window
+    if (has1 && wa >= 0 && wa + NF <= N && wb >= 0 && wb + NF <= N) {
+        // both windows inside their rows: no bounds checks
+        const float *pa = REV ? xa + (N - 1 - wa) : xa + wa;
+        const float *pb = REV ? xb + (N - 1 - wb) : xb + wb;
+#pragma unroll 4
+        for (int i = tid; i < NF; i += T)
+            d[pad(i)] = make_float2(pa[REV ? -i : i], pb[REV ? -i : i]);
+    } else {
+        for (int i = tid; i < NF; i += T) {
+            const int64_t ta = wa + i, tb = wb + i;
+            float va = 0.0f, vb = 0.0f;
+            if (ta >= 0 && ta < N) va = xa[REV ? N - 1 - ta : ta];
+            if (has1 && tb >= 0 && tb < N) vb = xb[REV ? N - 1 - tb : tb];
+            d[pad(i)] = make_float2(va, vb);
+        }
     }
     __syncthreads();
     fft<LOGN, false, false>(d, tq, nullptr, tid);
     fft<LOGN, true, true>(d, tq, H, tid);
     float *ya = y + ra * ldy, *yb = y + rb * ldy;
-    for (int v = tid; v < V; v += T) {
-        const float2 o = d[pad(K - 1 + v)];
-        const int64_t ta = oa + v, tb = ob + v;
-        if (ta < N) ya[REV ? N - 1 - ta : ta] = o.x;
-        if (has1 && tb < N) yb[REV ? N - 1 - tb : tb] = o.y;
+    if (has1 && oa + V <= N && ob + V <= N) {
+        float *qa = REV ? ya + (N - 1 - oa) : ya + oa;
+        float *qb = REV ? yb + (N - 1 - ob) : yb + ob;
+#pragma unroll 4
+        for (int v = tid; v < V; v += T) {
+            const float2 o = d[pad(K - 1 + v)];
+            qa[REV ? -v : v] = o.x;
+            qb[REV ? -v : v] = o.y;
+        }
+    } else {
+        for (int v = tid; v < V; v += T) {
+            const float2 o = d[pad(K - 1 + v)];
+            const int64_t ta = oa + v, tb = ob + v;
+            if (ta < N) ya[REV ? N - 1 - ta : ta] = o.x;
+            if (has1 && tb < N) yb[REV ? N - 1 - tb : tb] = o.y;
+        }
     }
 }
 
@@ -277,7 +315,8 @@ int twiddle_table(DeviceState *st, int logn, hipStream_t s, const float2 **out)
             cs != hipStreamCaptureStatusNone)
             return fail(SCIR_HIP_ERR_CAPTURE,
                         "the FFT twiddle table is built on first use; run once outside capture");
-        std::vector<float2> h(kFine + NF / kFine);
+        const int r01 = 16 * (logn == 14 ? 32 : 16);  // Plan<logn>::R0 * R1
+        std::vector<float2> h(kFine + NF / kFine + r01);
         for (int i = 0; i < kFine; ++i) {
             const double a = 2.0 * kPi * (double)i / (double)NF;
             h[i] = make_float2((float)std::cos(a), (float)std::sin(a));
@@ -285,6 +324,10 @@ int twiddle_table(DeviceState *st, int logn, hipStream_t s, const float2 **out)
         for (int i = 0; i < NF / kFine; ++i) {
             const double a = 2.0 * kPi * (double)(i * kFine) / (double)NF;
             h[kFine + i] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        for (int m = 0; m < r01; ++m) {
+            const double a = 2.0 * kPi * (double)m / (double)r01;
+            h[kFine + NF / kFine + m] = make_float2((float)std::cos(a), (float)std::sin(a));
         }
         SCIR_HIP_TRY(hipMalloc(&buf.ptr, h.size() * sizeof(float2)));
         buf.bytes = h.size() * sizeof(float2);
