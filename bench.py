@@ -192,9 +192,10 @@ def roofline_for(wl, name, prec, avg_launch_s):
         # HBM-bound (the FFT path's work per sample is O(log K))
         alg = 8.0 * samples
         ach = alg / avg_launch_s / 1e9
+        fam = "fft" if prec == "fft32" else "direct"
         return dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4),
-                    traffic=load_traffic(f"fir_direct_{prec}_{name}_pmc"))
+                    traffic=load_traffic(f"fir_{fam}_{prec}_{name}_pmc"))
     ach = flops / avg_launch_s / 1e12
     return dict(bound="fp32", achieved=round(ach, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                 frac=round(ach / FP32_PEAK_TFLOPS, 4),
