@@ -92,17 +92,20 @@ __device__ __forceinline__ double df2t_step(double v, double (&s1)[SMAX], double
     return v;
 }
 
-// Layout of the matrices sos_setup_kernel writes (d = 2S):
-//   C [16][d]            zero-input output k from unit state j
-//   P [kPowers][d][d]    (A^16)^(2^k)
+// Layout of the matrices sos_setup_kernel writes, D = 2*SMAX (d = 2S <= D;
+// entries beyond d are zero):
+//   C [16][D]            zero-input output k from unit state j
+//   P [kPowers][D][D]    (A^16)^(2^k)
 //   T [d][d]             A^L (the chunk transition; when there are chunks)
-__host__ __device__ constexpr int64_t mats_c(int d) { return 0; }
-__host__ __device__ constexpr int64_t mats_p(int d) { return (int64_t)kGroup * d; }
-__host__ __device__ constexpr int64_t mats_t(int d)
+// The wave kernel reads C and P with uniform indices, which the compiler
+// turns into scalar loads: the matrices sit in the scalar cache and reach
+// the multiply-adds as scalar operands.
+__host__ __device__ constexpr int64_t mats_p(int D) { return (int64_t)kGroup * D; }
+__host__ __device__ constexpr int64_t mats_t(int D)
 {
-    return (int64_t)kGroup * d + (int64_t)kPowers * d * d;
+    return (int64_t)kGroup * D + (int64_t)kPowers * D * D;
 }
-__host__ __device__ constexpr int64_t mats_size(int d) { return mats_t(d) + (int64_t)d * d; }
+__host__ __device__ constexpr int64_t mats_size(int D, int d) { return mats_t(D) + (int64_t)d * d; }
 
 // One workgroup.  q = L/16 when a chunk transition is needed, else 0.
 template <int SMAX>
@@ -113,6 +116,9 @@ __global__ __launch_bounds__(kThreads) void sos_setup_kernel(const double *__res
     __shared__ double base[D][D], res[D][D], tpr[D][D], tsq[D][D];
     const int d = 2 * S;
     const int tid = threadIdx.x;
+    if (tid >= d && tid < D) {  // padding columns of C
+        for (int k = 0; k < kGroup; ++k) mats[k * D + tid] = 0.0;
+    }
     if (tid < d) {
         // 16 zero-input steps from unit state tid: C's column and A^16's
         Coefs<SMAX> cf;
@@ -124,7 +130,7 @@ __global__ __launch_bounds__(kThreads) void sos_setup_kernel(const double *__res
             s2[i] = (2 * i + 1 == tid) ? 1.0 : 0.0;
         }
         for (int k = 0; k < kGroup; ++k)
-            mats[mats_c(d) + k * d + tid] = df2t_step<SMAX>(0.0, s1, s2, cf, S);
+            mats[k * D + tid] = df2t_step<SMAX>(0.0, s1, s2, cf, S);
 #pragma unroll
         for (int i = 0; i < SMAX; ++i) {
             if (i < S) {
@@ -134,9 +140,10 @@ __global__ __launch_bounds__(kThreads) void sos_setup_kernel(const double *__res
         }
     }
     __syncthreads();
-    for (int e = tid; e < d * d; e += kThreads) {
-        res[e / d][e % d] = (e / d == e % d) ? 1.0 : 0.0;
-        mats[mats_p(d) + e] = base[e / d][e % d];
+    for (int e = tid; e < d * d; e += kThreads) res[e / d][e % d] = (e / d == e % d) ? 1.0 : 0.0;
+    for (int e = tid; e < D * D; e += kThreads) {
+        const int i = e / D, j = e % D;
+        mats[mats_p(D) + e] = (i < d && j < d) ? base[i][j] : 0.0;
     }
     __syncthreads();
     // base runs through (A^16)^(2^k); res collects A^L = (A^16)^q bit by bit.
@@ -158,13 +165,18 @@ __global__ __launch_bounds__(kThreads) void sos_setup_kernel(const double *__res
             const int i = e / d, j = e % d;
             if (bit) res[i][j] = tpr[i][j];
             if (more) base[i][j] = tsq[i][j];
-            if (more && k + 1 < kPowers) mats[mats_p(d) + (int64_t)(k + 1) * d * d + e] = tsq[i][j];
+        }
+        if (more && k + 1 < kPowers) {  // padded to D x D with zeros
+            for (int e = tid; e < D * D; e += kThreads) {
+                const int i = e / D, j = e % D;
+                mats[mats_p(D) + (int64_t)(k + 1) * D * D + e] = (i < d && j < d) ? tsq[i][j] : 0.0;
+            }
         }
         __syncthreads();
         if (!more) break;
     }
     if (q > 0)
-        for (int e = tid; e < d * d; e += kThreads) mats[mats_t(d) + e] = res[e / d][e % d];
+        for (int e = tid; e < d * d; e += kThreads) mats[mats_t(D) + e] = res[e / d][e % d];
 }
 
 __device__ __forceinline__ void wave_sync()
@@ -177,7 +189,7 @@ __device__ __forceinline__ void wave_sync()
 // Inclusive scan over the wave of the lanes' end states: lane l ends with
 // sum_{j <= l} (A^16)^(l-j) e_j.
 template <int D>
-__device__ __forceinline__ void wave_scan(double (&e)[D], const double (*pw)[D][D], int lane)
+__device__ __forceinline__ void wave_scan(double (&e)[D], const double *__restrict__ pw, int lane)
 {
 #pragma unroll 1
     for (int k = 0; k < kPowers; ++k) {
@@ -190,7 +202,7 @@ __device__ __forceinline__ void wave_scan(double (&e)[D], const double (*pw)[D][
             for (int i = 0; i < D; ++i) {
                 double acc = e[i];
 #pragma unroll
-                for (int m = 0; m < D; ++m) acc = fma(pw[k][i][m], w[m], acc);
+                for (int m = 0; m < D; ++m) acc = fma(pw[(k * D + i) * D + m], w[m], acc);
                 e[i] = acc;
             }
         }
@@ -248,19 +260,8 @@ __global__ __launch_bounds__(kThreads) void sos_wave_kernel(
     const double *__restrict__ mats, double *__restrict__ zs, int vec)
 {
     constexpr int D = 2 * SMAX;
-    __shared__ double cm[kGroup][D];
-    __shared__ double pw[kPowers][D][D];
     __shared__ __attribute__((aligned(16))) double tiles[kThreads / 64][kStep];
     const int d = 2 * S;
-    for (int e = threadIdx.x; e < kGroup * D; e += kThreads) {
-        const int k = e / D, j = e % D;
-        cm[k][j] = j < d ? mats[mats_c(d) + k * d + j] : 0.0;
-    }
-    for (int e = threadIdx.x; e < kPowers * D * D; e += kThreads) {
-        const int k = e / (D * D), i = (e / D) % D, j = e % D;
-        pw[k][i][j] = (i < d && j < d) ? mats[mats_p(d) + (int64_t)k * d * d + i * d + j] : 0.0;
-    }
-    __syncthreads();
 
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
@@ -300,12 +301,12 @@ __global__ __launch_bounds__(kThreads) void sos_wave_kernel(
     const int pj = ((lane & 7) - mline) & 7;
 
     for (int64_t st = 0; st < nsteps; ++st) {
-        // The matrices are re-read from LDS every step: an opaque zero
+        // The matrices are re-read (scalar loads) every step: an opaque zero
         // offset keeps the compiler from holding all of them in registers.
         int zoff = 0;
         asm volatile("" : "+s"(zoff));
-        const double(*cmv)[D] = cm + zoff;
-        const double(*pwv)[D][D] = pw + zoff;
+        const double *__restrict__ cmv = mats + zoff;
+        const double *__restrict__ pwv = mats + mats_p(D) + zoff;
         int64_t lo, cnt;
         group_span<REV>(st * 64 + lane, ngroups, p0, p1, &lo, &cnt);
         const bool full = cnt == kGroup;
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(kThreads) void sos_wave_kernel(
             for (int i = 0; i < D; ++i) {
                 double acc = e[i];
 #pragma unroll
-                for (int m = 0; m < D; ++m) acc = fma(pwv[0][i][m], zin[m], acc);
+                for (int m = 0; m < D; ++m) acc = fma(pwv[i * D + m], zin[m], acc);
                 e[i] = acc;
             }
         }
@@ -392,7 +393,7 @@ __global__ __launch_bounds__(kThreads) void sos_wave_kernel(
                     if (k < cnt) {
                         double acc = v[k];
 #pragma unroll
-                        for (int m = 0; m < D; ++m) acc = fma(cmv[tk][m], z[m], acc);
+                        for (int m = 0; m < D; ++m) acc = fma(cmv[tk * D + m], z[m], acc);
                         v[k] = acc;
                     }
                 }
@@ -576,7 +577,7 @@ int sos_launch_t(const double *x, int64_t ldx, double *y, int64_t ldy, int64_t r
                            x, ldx, y, ldy, rows, N, sos, S, L, nC, mats, zs, vec);
         SCIR_TRY(check_launch("sos_wave_kernel(state)"));
         hipLaunchKernelGGL((sos_scan_kernel<SMAX>), dim3((unsigned)((rows + 3) / 4)),
-                           dim3(kThreads), 0, s, zs, mats + mats_t(2 * S), rows, nC, S);
+                           dim3(kThreads), 0, s, zs, mats + mats_t(2 * SMAX), rows, nC, S);
         SCIR_TRY(check_launch("sos_scan_kernel"));
     }
     hipLaunchKernelGGL((sos_wave_kernel<SMAX, kRun, REV>), dim3(blocks), dim3(kThreads), 0, s, x,
@@ -647,7 +648,8 @@ int sos_run(const double *x, int64_t ldx, double *y, int64_t ldy, int64_t rows, 
     if (!row_mode) {
         StreamScratch &sc = scratch_for(st, s);
         const int d = 2 * S;
-        const int64_t mdoubles = (mats_size(d) + 31) / 32 * 32;
+        const int Dk = 2 * (S <= 8 ? S : kMaxSections);  // the kernels' padded dimension
+        const int64_t mdoubles = (mats_size(Dk, d) + 31) / 32 * 32;
         const int64_t zdoubles = nC > 1 ? rows * nC * d : 0;
         SCIR_TRY(ensure_buffer(sc.sos_states, (size_t)(mdoubles + zdoubles) * sizeof(double), s));
         mats = static_cast<double *>(sc.sos_states.ptr);
