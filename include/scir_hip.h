@@ -287,6 +287,11 @@ SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
  *   "mfma.segs"       row segments per workgroup of the MFMA kernel (>= 1); the
  *                     next segment's samples load while the current one computes
  *   "auto.mfma_min_k" smallest K for which AUTO picks BF16x3 MFMA
+ *   "auto.fft"        1: AUTO may pick the FFT32 path (normwise error) from
+ *                     auto.fft_min_k up (K <= 8192, x != y); 0 (default): never
+ *   "auto.fft_min_k"  smallest K for which AUTO (with auto.fft) picks FFT32;
+ *                     the autotuner measures it (the K from which FFT32 beats
+ *                     BF16x3 MFMA on long rows)
  * The autotuner's search uses the same mechanism. */
 SCIR_HIP_API int scir_hip_tune_set(const char *key, int32_t value);
 

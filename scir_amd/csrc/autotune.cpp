@@ -6,12 +6,15 @@
 //                                                 prefetch, 2: LDS DMA)
 //   mfma_tiles / mfma_tiles_fp16                  MFMA kernel tiles per wave
 //   mfma_segs                                     MFMA row segments per workgroup
-// plus one global AUTO cut-over K (F32 direct below, BF16x3 MFMA from there).
+// plus one global AUTO cut-over K (F32 direct below, BF16x3 MFMA from there)
+// and the K from which the FP32 FFT path beats BF16x3 MFMA (auto_fft_min_k;
+// AUTO takes the FFT path from there only when the tuning key auto.fft is 1,
+// since its error is normwise rather than elementwise).
 //
 // scir_hip_autotune() fills it by timing every candidate with HIP events on
 // synthetic rows (2^26 samples per bucket, N = the bucket's representative
 // row length) on the current device, keeps the fastest, and writes
-//   {"arch": "gfx950", "auto_mfma_min_k": 160,
+//   {"arch": "gfx950", "auto_mfma_min_k": 160, "auto_fft_min_k": 512,
 //    "entries": [{"k_lo": 1, "k_hi": 16, "n_lo": 0, "n_hi": 8192,
 //                 "direct_r": 8, "direct_loader": 1, "direct_seg_tiles": 2,
 //                 "mfma_tiles": 2, "mfma_tiles_fp16": 4, "mfma_segs": 16}, ...]}
@@ -39,6 +42,8 @@ struct Entry {
 std::mutex g_tune_mu;
 std::vector<Entry> g_table;
 int g_auto_min_k = -1;  // from the table; -1 = built-in default
+int g_auto_fft_min_k = -1;  // from the table; -1 = built-in default
+int g_over_auto_fft = -1, g_over_auto_fft_min_k = -1;
 // Overrides set through scir_hip_tune_set (-1 = unset).
 int g_over_direct_r = -1, g_over_direct_loader = -1, g_over_direct_seg = -1,
     g_over_mfma_tiles = -1, g_over_auto_min_k = -1, g_over_mfma_segs = -1,
@@ -63,6 +68,9 @@ TuneChoice tune_lookup(int K, int64_t N)
             break;
         }
     if (g_auto_min_k >= 0) c.auto_mfma_min_k = g_auto_min_k;
+    if (g_auto_fft_min_k >= 0) c.auto_fft_min_k = g_auto_fft_min_k;
+    if (g_over_auto_fft >= 0) c.auto_fft = g_over_auto_fft != 0;
+    if (g_over_auto_fft_min_k >= 0) c.auto_fft_min_k = g_over_auto_fft_min_k;
     if (g_over_direct_r >= 0) c.direct_r = g_over_direct_r;
     if (g_over_direct_loader >= 0) c.direct_loader = g_over_direct_loader;
     if (g_over_direct_seg >= 0) c.direct_seg_tiles = g_over_direct_seg;
@@ -116,6 +124,8 @@ std::string table_json(const std::string &arch)
     std::ostringstream o;
     o << "{\"arch\": \"" << arch << "\", \"auto_mfma_min_k\": "
       << (g_auto_min_k >= 0 ? g_auto_min_k : TuneChoice{}.auto_mfma_min_k)
+      << ", \"auto_fft_min_k\": "
+      << (g_auto_fft_min_k >= 0 ? g_auto_fft_min_k : TuneChoice{}.auto_fft_min_k)
       << ",\n \"entries\": [";
     for (size_t i = 0; i < g_table.size(); ++i) {
         const Entry &e = g_table[i];
@@ -189,6 +199,11 @@ extern "C" int scir_hip_tune_set(const char *key, int32_t value)
         g_over_mfma_tiles = v;
     } else if (!std::strcmp(key, "auto.mfma_min_k")) {
         g_over_auto_min_k = v;
+    } else if (!std::strcmp(key, "auto.fft")) {
+        if (v > 1) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "auto.fft must be 0 or 1");
+        g_over_auto_fft = v;
+    } else if (!std::strcmp(key, "auto.fft_min_k")) {
+        g_over_auto_fft_min_k = v;
     } else if (!std::strcmp(key, "mfma.segs")) {
         if (v == 0) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "mfma.segs must be >= 1");
         g_over_mfma_segs = v;
@@ -242,11 +257,13 @@ extern "C" int scir_hip_autotune_load(const char *cache_path)
         table.push_back(e);
         p = b;
     }
-    int64_t amk = -1;
+    int64_t amk = -1, afk = -1;
     find_int(s, 0, s.size(), "auto_mfma_min_k", &amk);
+    find_int(s, 0, s.size(), "auto_fft_min_k", &afk);  // optional
     std::lock_guard<std::mutex> g(g_tune_mu);
     g_table = table;
     g_auto_min_k = (int)amk;
+    g_auto_fft_min_k = (int)afk;
     return SCIR_HIP_OK;
 }
 
@@ -406,6 +423,29 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
         }
         if (cut < 0) cut = 1 << 20;
     }
+    // FFT cut-over: the smallest K at which FP32 FFT beats BF16x3 MFMA.
+    int fft_cut = TuneChoice{}.auto_fft_min_k;
+    if (status == SCIR_HIP_OK && !over_budget()) {
+        const int ks[] = {128, 192, 256, 384, 512, 768, 1024, 2048};
+        fft_cut = -1;
+        for (int K : ks) {
+            if (status != SCIR_HIP_OK) break;
+            const int64_t N = long_n, B = total / N;
+            float mf = 0, m3 = 0;
+            status = time_ms([&]() {
+                return scir_hip_fir_f32_dev(x, N, b, K, B, N, y, N, SCIR_HIP_PREC_FFT32, s);
+            }, s, 3, &mf);
+            if (status == SCIR_HIP_OK)
+                status = time_ms([&]() {
+                    return scir_hip_fir_f32_dev(x, N, b, K, B, N, y, N, SCIR_HIP_PREC_BF16X3, s);
+                }, s, 3, &m3);
+            if (status == SCIR_HIP_OK && mf < m3) {
+                fft_cut = K;
+                break;
+            }
+        }
+        if (fft_cut < 0) fft_cut = 1 << 20;
+    }
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         g_over_direct_r = saved[0];
@@ -417,6 +457,7 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
         if (status == SCIR_HIP_OK) {
             g_table = found;
             g_auto_min_k = cut;
+            g_auto_fft_min_k = fft_cut;
         }
     }
     cleanup();

@@ -111,6 +111,10 @@ struct TuneChoice {
     int mfma_segs = 16;       // row segments per workgroup of the MFMA kernel
     bool mfma_segs_forced = false;  // set through scir_hip_tune_set: no grid-size cap
     int auto_mfma_min_k = 56; // AUTO uses BF16x3 MFMA from this K up
+    // AUTO may take the FFT path (normwise error) from auto_fft_min_k up,
+    // when enabled with the tuning key auto.fft = 1 (off by default).
+    bool auto_fft = false;
+    int auto_fft_min_k = 512; // measured: FFT32 beats BF16x3 MFMA from here (K <= 8192)
 };
 TuneChoice tune_lookup(int K, int64_t N);
 

@@ -41,6 +41,11 @@ def test_overrides_validated():
     assert L.scir_hip_tune_set(b"direct.dma_bufs", -1) == _lib.OK
     assert L.scir_hip_tune_set(b"mfma.segs", 8) == _lib.OK
     assert L.scir_hip_tune_set(b"mfma.segs", -1) == _lib.OK
+    assert L.scir_hip_tune_set(b"auto.fft", 2) == _lib.ERR_INVALID_ARGUMENT
+    assert L.scir_hip_tune_set(b"auto.fft", 1) == _lib.OK
+    assert L.scir_hip_tune_set(b"auto.fft", -1) == _lib.OK
+    assert L.scir_hip_tune_set(b"auto.fft_min_k", 300) == _lib.OK
+    assert L.scir_hip_tune_set(b"auto.fft_min_k", -1) == _lib.OK
     assert L.scir_hip_tune_set(b"no.such.key", 1) == _lib.ERR_INVALID_ARGUMENT
     assert L.scir_hip_tune_set(b"direct.r", 16) == _lib.OK
     assert L.scir_hip_tune_set(b"direct.r", -1) == _lib.OK
