@@ -254,6 +254,15 @@ SCIR_HIP_API int scir_hip_fir_vjp_dx_dev(const float *g, int64_t ldg, const floa
 SCIR_HIP_API int scir_hip_fir_vjp_db_dev(const float *g, int64_t ldg, const float *x, int64_t ldx,
                             int32_t K, int64_t B, int64_t N, float *db, void *stream);
 
+/* Host tier of both adjoints (the Rust vjp API's entry): host arrays in and
+ * out, synchronous, like scir_hip_fir_f32_host.  dx [B, lddx >= N] and db[K]
+ * are overwritten; dx uses `precision` (as scir_hip_fir_vjp_dx_dev), db the
+ * deterministic FP32 reduction. */
+SCIR_HIP_API int scir_hip_fir_vjp_f32_host(const float *x, int64_t ldx, const float *g,
+                                           int64_t ldg, const float *b, int32_t K, int64_t B,
+                                           int64_t N, float *dx, int64_t lddx, float *db,
+                                           int32_t precision);
+
 /* ---- HIP graphs ------------------------------------------------------------
  * Capture any sequence of *_dev / ragged-plan calls enqueued on `stream`
  * between begin and end, then replay it.  Calls that would need to grow

@@ -71,6 +71,8 @@ SIGNATURES = {
     "scir_hip_filtfilt_f64_host": (_I, [_P, _I64, _I64, _I64, _P, _I32, _P, _I64, _I32]),
     "scir_hip_sosfilt_f64_cpu": (_I, [_P, _I64, _I64, _I64, _P, _I32, _P, _I64, _I32]),
     "scir_hip_filtfilt_f64_cpu": (_I, [_P, _I64, _I64, _I64, _P, _I32, _P, _I64, _I32]),
+    "scir_hip_fir_vjp_f32_host": (_I, [_P, _I64, _P, _I64, _P, _I32, _I64, _I64, _P, _I64, _P,
+                                      _I32]),
     "scir_hip_buffer_create": (_I, [_c.c_size_t, _c.POINTER(_P)]),
     "scir_hip_buffer_upload": (_I, [_P, _P, _c.c_size_t]),
     "scir_hip_buffer_download": (_I, [_P, _P, _c.c_size_t]),

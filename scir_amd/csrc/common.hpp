@@ -62,6 +62,7 @@ struct DeviceState {
     std::map<void *, StreamScratch> scratch;
     // Host tier: device copies of the caller's host arrays.
     DeviceBuffer host_x, host_y, host_b;
+    DeviceBuffer host_z;  // third matrix of the host-tier adjoint (dL/dx)
     hipStream_t host_stream = nullptr;
 };
 

@@ -121,6 +121,33 @@ def fir1d_batched_f32_hip(x, taps, precision="f32") -> np.ndarray:
     return y
 
 
+def fir1d_batched_f32_vjp_hip(x, taps, g, precision="f32"):
+    """Vector-Jacobian product of ``fir1d_batched_f32`` on the HIP device
+    (host arrays in and out): for L = sum(g * fir1d_batched_f32(x, taps))
+    returns ``(dL/dx, dL/dtaps)``, both in the caller's convention (dtaps in
+    the reference's taps order).  No counterpart in the reference."""
+    xa = _as_x(x)
+    ga = _as_x(g)
+    if ga.shape != xa.shape:
+        raise ShapeMismatch("g must have the shape of x")
+    b = _as_b(taps)
+    B, N = xa.shape
+    dx = np.zeros((B, N), np.float32)
+    db = np.zeros(b.size, np.float32)
+    try:
+        L = _lib.lib()
+    except _lib.LibraryMissing as e:
+        raise BackendUnavailable(str(e)) from None
+    status = L.scir_hip_fir_vjp_f32_host(_ptr(xa), max(N, 1), _ptr(ga), max(N, 1), _ptr(b), b.size,
+                                         B, N, _ptr(dx), max(N, 1), _ptr(db),
+                                         _lib.precision_code(precision))
+    if status == _lib.ERR_INVALID_ARGUMENT:
+        raise ShapeMismatch(L.scir_hip_last_error().decode())
+    if status != _lib.OK:
+        raise BackendUnavailable(L.scir_hip_last_error().decode())
+    return dx, db[::-1].copy()
+
+
 def fir1d_batched_f32_auto(x, taps, device: Device) -> np.ndarray:
     """FIR over each row of ``x`` with device dispatch
     (crates/scir-gpu/src/lib.rs:515-531).

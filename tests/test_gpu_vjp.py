@@ -163,3 +163,30 @@ def test_full_size_differentiation_workload(hip, oracle):
     torch.cuda.synchronize()
     lim = (N / 2048 + 200) * 2.0 ** -24 * 2.0 * sabs.cpu().numpy()
     assert np.all(np.abs(db_all.cpu().numpy() - (db_lo + db_hi).cpu().numpy()) <= lim)
+
+
+@pytest.mark.parametrize("precision", ["f32", "bf16x3", "fft32"])
+def test_host_tier_vjp(hip, oracle, precision):
+    """The host-tier entry (the Rust vjp API's): both gradients in the
+    caller's convention, equal to the device-tier kernels' results."""
+    from scir_amd.gpu import fir1d_batched_f32_vjp_hip
+    r = rng(77)
+    B, N, K = 3, 9001, 257
+    x = r.standard_normal((B, N)).astype(np.float32)
+    taps = r.standard_normal(K).astype(np.float32)
+    w = r.standard_normal((B, N)).astype(np.float32)
+    dx, dtaps = fir1d_batched_f32_vjp_hip(x, taps, w, precision)
+    b = taps[::-1].copy()                     # lfilter order
+    dx_dev, db_dev = gpu_grads(x, b, w, precision)
+    assert np.array_equal(dx, dx_dev)
+    assert np.array_equal(dtaps, db_dev[::-1])
+    dx64, db64 = oracle.vjp_f64(w, x, b)
+    bdx, bdb = bounds(oracle, x, b, w)
+    if precision != "fft32":
+        assert np.all(np.abs(dx - dx64) <= bdx)
+    assert oracle.rel_l2(dx, dx64) <= 1e-6
+    assert np.all(np.abs(dtaps - db64[::-1]) <= bdb[::-1])
+    # empty rows: dtaps = 0, dx empty
+    dx0, dt0 = fir1d_batched_f32_vjp_hip(np.zeros((2, 0), np.float32), taps,
+                                         np.zeros((2, 0), np.float32))
+    assert dx0.shape == (2, 0) and not np.any(dt0)
