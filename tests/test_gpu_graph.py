@@ -1,4 +1,5 @@
-"""HIP-graph capture of the dense device tier and the adjoints.
+"""HIP-graph capture of the dense device tier (every precision, FFT32
+included), the adjoints, sosfilt / filtfilt and the resampler.
 
 The library's kernels take every buffer as an argument and clear memory
 with kernels (never hipMemsetAsync), so a captured sequence replays the
@@ -19,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("precision,K", [("f32", 33), ("f32_exact", 33), ("bf16x3", 512),
-                                         ("fp16", 512), ("auto", 129)])
+                                         ("fp16", 512), ("auto", 129), ("fft32", 700)])
 def test_dense_graph_replays_match_eager(hip, precision, K):
     import torch
     from scir_amd import device
@@ -76,3 +77,55 @@ def test_growth_during_capture_is_refused(hip):
             device.vjp_db(x, x, 64)
     assert "captur" in str(ei.value).lower()
     torch.cuda.synchronize()
+
+
+def test_sosfilt_and_resample_graph_replays(hip):
+    """The float64 paths under capture: sosfilt (setup, chunk-state, scan and
+    filtering launches, few rows so that rows split into chunks), filtfilt in
+    place, and the resampler replay their eager results bit for bit."""
+    import ctypes
+    import torch
+    from scipy import signal as ss
+    from scir_amd import _lib, device, signal
+    from scir_amd.ragged import Graph
+    r = rng(91)
+    rows, n = 3, 300000
+    x = torch.tensor(r.standard_normal((rows, n)), device="cuda")
+    sos = torch.tensor(ss.butter(6, 0.15, output="sos"), device="cuda")
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    h = torch.tensor(signal.RESAMPLE_H_2_3, device="cuda")
+    m = signal.resample_len(n, 2, 3, h.numel())
+    w = torch.empty((rows, m), dtype=torch.float64, device="cuda")
+    L = _lib.lib()
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())
+    s = torch.cuda.Stream()
+
+    def work():
+        device.sosfilt(x, sos, out=y)
+        z.copy_(x)
+        device.sosfilt(z, sos, zero_phase=True, out=z)
+        _lib.check(L.scir_hip_resample_poly_f64_dev(
+            vp(x), n, rows, n, 2, 3, vp(h), h.numel(), vp(w), m,
+            ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    with torch.cuda.stream(s):
+        work()  # eager first: the chunk-state scratch is allocated here
+    s.synchronize()
+    torch.cuda.synchronize()
+    with Graph(s) as gr:
+        with torch.cuda.stream(s):
+            work()
+    for rep in range(3):
+        x.copy_(torch.tensor(r.standard_normal((rows, n)), device="cuda"))
+        work()
+        ye, ze, we = y.clone(), z.clone(), w.clone()
+        for t in (y, z, w):
+            t.fill_(float("nan"))
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            gr.launch()
+        s.synchronize()
+        assert torch.equal(y, ye), f"sosfilt replay {rep} differs"
+        assert torch.equal(z, ze), f"filtfilt replay {rep} differs"
+        assert torch.equal(w, we), f"resample replay {rep} differs"
+    gr.close()
