@@ -429,15 +429,20 @@ __global__ __launch_bounds__(kThreads) void sos_wave_kernel(
 }
 
 // One lane per row, the reference loop exactly (reference order, and rows
-// too short to split).
+// too short to split).  A wave serves `rpw` rows (its other lanes idle), so
+// that few rows still spread over many CUs, and each lane keeps kRowAhead
+// groups of 16 samples in flight (the recursion itself is sequential).
+constexpr int kRowAhead = 4;
+
 template <int SMAX, bool REV>
-__global__ __launch_bounds__(kThreads) void sos_row_kernel(const double *x, int64_t ldx,
-                                                           double *y, int64_t ldy, int64_t rows,
-                                                           int64_t N,
-                                                           const double *__restrict__ sos, int S,
-                                                           int vec)
+__global__ __launch_bounds__(64) void sos_row_kernel(const double *x, int64_t ldx, double *y,
+                                                     int64_t ldy, int64_t rows, int64_t N,
+                                                     const double *__restrict__ sos, int S,
+                                                     int vec, int rpw)
 {
-    const int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int lane = threadIdx.x;
+    if (lane >= rpw) return;
+    const int64_t row = (int64_t)blockIdx.x * rpw + lane;
     if (row >= rows) return;
     Coefs<SMAX> cf;
     load_coefs(cf, sos, S);
@@ -446,23 +451,59 @@ __global__ __launch_bounds__(kThreads) void sos_row_kernel(const double *x, int6
     for (int i = 0; i < SMAX; ++i) s1[i] = s2[i] = 0.0;
     const double *xr = x + row * ldx;
     double *yr = y + row * ldy;
-    if (!REV) {
-        int64_t t = 0;
-        if (vec) {
-            for (; t + kGroup <= N; t += kGroup) {
-                double v[kGroup];
-                load_lane(v, xr, t, kGroup, true);
+    // Whole groups [16m, 16m+16) for m < nfull; back to front the samples
+    // above the last whole group come first in time.
+    const int64_t nfull = vec ? N / kGroup : 0;
+    if (REV)
+        for (int64_t t = N - 1; t >= nfull * kGroup; --t) yr[t] = df2t_step<SMAX>(xr[t], s1, s2, cf, S);
+    if (nfull > 0) {
+        // ring[j] holds group number (in time) g + j
+        double2 ring[kRowAhead][kGroup / 2];
+        auto gbase = [&](int64_t g) { return (REV ? nfull - 1 - g : g) * kGroup; };
 #pragma unroll
-                for (int k = 0; k < kGroup; ++k) v[k] = df2t_step<SMAX>(v[k], s1, s2, cf, S);
+        for (int j = 0; j < kRowAhead; ++j)
+            if (j < nfull) {
 #pragma unroll
                 for (int k = 0; k < kGroup / 2; ++k)
-                    *reinterpret_cast<double2 *>(yr + t + 2 * k) = make_double2(v[2 * k], v[2 * k + 1]);
+                    ring[j][k] = *reinterpret_cast<const double2 *>(xr + gbase(j) + 2 * k);
+            }
+        for (int64_t g = 0; g < nfull; g += kRowAhead) {
+#pragma unroll
+            for (int j = 0; j < kRowAhead; ++j) {
+                if (g + j < nfull) {
+                    double v[kGroup];
+#pragma unroll
+                    for (int k = 0; k < kGroup / 2; ++k) {
+                        v[2 * k] = ring[j][k].x;
+                        v[2 * k + 1] = ring[j][k].y;
+                    }
+                    // refill this slot with the group kRowAhead later (read
+                    // before any write of it: in place is safe)
+                    if (g + j + kRowAhead < nfull) {
+#pragma unroll
+                        for (int k = 0; k < kGroup / 2; ++k)
+                            ring[j][k] = *reinterpret_cast<const double2 *>(
+                                xr + gbase(g + j + kRowAhead) + 2 * k);
+                    }
+                    if (!REV) {
+#pragma unroll
+                        for (int k = 0; k < kGroup; ++k) v[k] = df2t_step<SMAX>(v[k], s1, s2, cf, S);
+                    } else {
+#pragma unroll
+                        for (int k = kGroup - 1; k >= 0; --k)
+                            v[k] = df2t_step<SMAX>(v[k], s1, s2, cf, S);
+                    }
+                    const int64_t base = gbase(g + j);
+#pragma unroll
+                    for (int k = 0; k < kGroup / 2; ++k)
+                        *reinterpret_cast<double2 *>(yr + base + 2 * k) =
+                            make_double2(v[2 * k], v[2 * k + 1]);
+                }
             }
         }
-        for (; t < N; ++t) yr[t] = df2t_step<SMAX>(xr[t], s1, s2, cf, S);
-    } else {
-        for (int64_t t = N - 1; t >= 0; --t) yr[t] = df2t_step<SMAX>(xr[t], s1, s2, cf, S);
     }
+    if (!REV)
+        for (int64_t t = nfull * kGroup; t < N; ++t) yr[t] = df2t_step<SMAX>(xr[t], s1, s2, cf, S);
 }
 
 // Exclusive scan over the chunks of each row, in place on zs:
@@ -559,12 +600,14 @@ bool aligned_rows(const double *p, int64_t ld)
 template <int SMAX, bool REV>
 int sos_launch_t(const double *x, int64_t ldx, double *y, int64_t ldy, int64_t rows, int64_t N,
                  const double *sos, int S, bool row_mode, int64_t L, int64_t nC, double *mats,
-                 double *zs, hipStream_t s)
+                 double *zs, int cu_count, hipStream_t s)
 {
     const int vec = aligned_rows(x, ldx) && aligned_rows(y, ldy);
     if (row_mode) {
-        hipLaunchKernelGGL((sos_row_kernel<SMAX, REV>), dim3((unsigned)((rows + kThreads - 1) / kThreads)),
-                           dim3(kThreads), 0, s, x, ldx, y, ldy, rows, N, sos, S, vec);
+        // rows per wave: spread the rows over the CUs (one wave each) first
+        const int rpw = (int)std::min<int64_t>(64, std::max<int64_t>(1, (rows + cu_count - 1) / cu_count));
+        hipLaunchKernelGGL((sos_row_kernel<SMAX, REV>), dim3((unsigned)((rows + rpw - 1) / rpw)),
+                           dim3(64), 0, s, x, ldx, y, ldy, rows, N, sos, S, vec, rpw);
         return check_launch("sos_row_kernel");
     }
     hipLaunchKernelGGL((sos_setup_kernel<SMAX>), dim3(1), dim3(kThreads), 0, s, sos, S,
@@ -588,18 +631,18 @@ int sos_launch_t(const double *x, int64_t ldx, double *y, int64_t ldy, int64_t r
 template <bool REV>
 int sos_launch_rev(const double *x, int64_t ldx, double *y, int64_t ldy, int64_t rows, int64_t N,
                    const double *sos, int S, bool row_mode, int64_t L, int64_t nC, double *mats,
-                   double *zs, hipStream_t s)
+                   double *zs, int cu_count, hipStream_t s)
 {
 #define SCIR_SOS_CASE(n)                                                                     \
     case n:                                                                                  \
         return sos_launch_t<n, REV>(x, ldx, y, ldy, rows, N, sos, S, row_mode, L, nC, mats, \
-                                    zs, s);
+                                    zs, cu_count, s);
     switch (S) {
         SCIR_SOS_CASE(1) SCIR_SOS_CASE(2) SCIR_SOS_CASE(3) SCIR_SOS_CASE(4)
         SCIR_SOS_CASE(5) SCIR_SOS_CASE(6) SCIR_SOS_CASE(7) SCIR_SOS_CASE(8)
         default:
             return sos_launch_t<kMaxSections, REV>(x, ldx, y, ldy, rows, N, sos, S, row_mode, L,
-                                                   nC, mats, zs, s);
+                                                   nC, mats, zs, cu_count, s);
     }
 #undef SCIR_SOS_CASE
 }
@@ -642,7 +685,7 @@ int sos_run(const double *x, int64_t ldx, double *y, int64_t ldy, int64_t rows, 
     bool row_mode = false;
     int64_t L = 0, nC = 0;
     sos_plan(rows, N, st->cu_count, reference_order, &row_mode, &L, &nC);
-    if ((row_mode ? rows / kThreads : rows * nC / (kThreads / 64)) + 2 > 0x7fffffffLL)
+    if ((row_mode ? rows : rows * nC / (kThreads / 64)) + 2 > 0x7fffffffLL)
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "too many rows");
     double *mats = nullptr, *zs = nullptr;
     if (!row_mode) {
@@ -656,8 +699,10 @@ int sos_run(const double *x, int64_t ldx, double *y, int64_t ldy, int64_t rows, 
         zs = mats + mdoubles;
     }
     if (rev)
-        return sos_launch_rev<true>(x, ldx, y, ldy, rows, N, sos, S, row_mode, L, nC, mats, zs, s);
-    return sos_launch_rev<false>(x, ldx, y, ldy, rows, N, sos, S, row_mode, L, nC, mats, zs, s);
+        return sos_launch_rev<true>(x, ldx, y, ldy, rows, N, sos, S, row_mode, L, nC, mats, zs,
+                                    st->cu_count, s);
+    return sos_launch_rev<false>(x, ldx, y, ldy, rows, N, sos, S, row_mode, L, nC, mats, zs,
+                                 st->cu_count, s);
 }
 
 int sos_args(const double *x, int64_t ldx, int64_t rows, int64_t N, const double *sos, int32_t S,
