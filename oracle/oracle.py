@@ -16,8 +16,14 @@ Functions and the reference lines they restate:
 * :func:`lfilter_ragged_f64` -- the CSR batch form of :func:`lfilter_f64`.
 * :func:`resample_poly_f64` -- crates/scir-signal/src/lib.rs:293-362
   (``convolve`` + ``resample_poly``), for any up/down and taps.
+* :func:`sosfilt_f64` / :func:`filtfilt_f64` -- crates/scir-signal/src/lib.rs:
+  260-291 over crates/scir-iir-filters/src/filter.rs:47-62 (Direct Form II
+  Transposed), bit-faithful.
+* :func:`sosfilt_longdouble` -- the same cascade in x87 extended precision,
+  the near-exact result the float64 loop's own error is measured against.
 
-Every function is pinned by tests/test_oracle.py (see the C file header).
+Every function is pinned by tests/test_oracle.py, tests/test_resample.py or
+tests/test_sosfilt.py (see the C file header).
 """
 from __future__ import annotations
 
