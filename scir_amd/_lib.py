@@ -115,6 +115,15 @@ def lib():
             raise LibraryMissing(
                 f"{LIB_PATH} is not built; run `python -m scir_amd.build` "
                 "(no CPU or torch fallback exists for the HIP path)")
+        # One HIP runtime per process: torch ships its own libamdhip64 with
+        # the same soname as the system one the library links.  Whichever is
+        # loaded first serves both, and torch only works on its own build, so
+        # when torch is importable it is loaded before the library (device
+        # pointers and streams then come from the runtime the kernels use).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)

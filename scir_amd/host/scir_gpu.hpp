@@ -13,6 +13,12 @@
 //                                    crates/scir-gpu/src/lib.rs:515-531
 //   scir_signal::gpu::fir1d_batched_f32(x, taps, device)
 //                                    crates/scir-signal/src/lib.rs:364-375
+//   scir_signal::sosfilt / filtfilt(sos, x, device)
+//                                    crates/scir-signal/src/lib.rs:260-291
+//   scir_signal::resample_poly(x, up, down, device)
+//                                    crates/scir-signal/src/lib.rs:293-362
+//   scir_gpu::fir1d_batched_f32_vjp_hip(x, taps, g)
+//                                    no counterpart (the adjoints)
 //
 // `taps` is in the reference's order (y[i] = sum_t taps[K-1-t] x[i-t]); the C
 // ABI takes lfilter-order b = reversed taps, so the reversal happens here.
@@ -31,6 +37,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/scir_hip.h"
@@ -169,4 +176,98 @@ inline scir_gpu::Array2<float> fir1d_batched_f32(const scir_gpu::ArrayView2<floa
 }
 
 }  // namespace gpu
+
+// One second-order section {b0, b1, b2, a0, a1, a2}: the layout of the
+// reference's #[repr(C)] SosCoeffs (crates/scir-iir-filters/src/sos.rs);
+// a0 is not read, as in the reference.
+struct SosCoeffs {
+    double b0, b1, b2, a0, a1, a2;
+};
+using Sos = std::vector<SosCoeffs>;
+
+namespace detail {
+
+inline std::vector<double> sos_filter(const Sos &sos, const std::vector<double> &x,
+                                      scir_gpu::Device device, bool zero_phase)
+{
+    std::vector<double> y(x.size());
+    if (x.empty()) return y;
+    const double *c = sos.empty() ? nullptr : &sos[0].b0;
+    const int64_t n = (int64_t)x.size();
+    int st;
+    if (device == scir_gpu::Device::Cpu)
+        st = zero_phase ? scir_hip_filtfilt_f64_cpu(x.data(), n, 1, n, c, (int32_t)sos.size(),
+                                                    y.data(), n, 1)
+                        : scir_hip_sosfilt_f64_cpu(x.data(), n, 1, n, c, (int32_t)sos.size(),
+                                                   y.data(), n, 1);
+    else
+        st = zero_phase ? scir_hip_filtfilt_f64_host(x.data(), n, 1, n, c, (int32_t)sos.size(),
+                                                     y.data(), n, 0)
+                        : scir_hip_sosfilt_f64_host(x.data(), n, 1, n, c, (int32_t)sos.size(),
+                                                    y.data(), n, 0);
+    if (st != SCIR_HIP_OK) throw scir_gpu::detail::error_from(st);
+    return y;
+}
+
+}  // namespace detail
+
+// sosfilt / filtfilt (crates/scir-signal/src/lib.rs:260-291) with the device
+// choice added; the CPU arm is the reference loop, bit-identical.
+inline std::vector<double> sosfilt(const Sos &sos, const std::vector<double> &x,
+                                   scir_gpu::Device device = scir_gpu::Device::Cpu)
+{
+    return detail::sos_filter(sos, x, device, false);
+}
+
+inline std::vector<double> filtfilt(const Sos &sos, const std::vector<double> &x,
+                                    scir_gpu::Device device = scir_gpu::Device::Cpu)
+{
+    return detail::sos_filter(sos, x, device, true);
+}
+
+// resample_poly (crates/scir-signal/src/lib.rs:293-362) for any up, down and
+// filter h, with the device choice added.
+inline std::vector<double> resample_poly(const std::vector<double> &x, int up, int down,
+                                         const std::vector<double> &h,
+                                         scir_gpu::Device device = scir_gpu::Device::Cpu)
+{
+    int64_t m = 0;
+    const int64_t n = (int64_t)x.size();
+    int st = scir_hip_resample_len(n, up, down, (int32_t)h.size(), &m);
+    if (st != SCIR_HIP_OK) throw scir_gpu::detail::error_from(st);
+    std::vector<double> y((size_t)m);
+    if (m == 0 || n == 0) return y;
+    st = device == scir_gpu::Device::Cpu
+             ? scir_hip_resample_poly_f64_cpu(x.data(), n, 1, n, up, down, h.data(),
+                                              (int32_t)h.size(), y.data(), m, 1)
+             : scir_hip_resample_poly_f64_host(x.data(), n, 1, n, up, down, h.data(),
+                                               (int32_t)h.size(), y.data(), m);
+    if (st != SCIR_HIP_OK) throw scir_gpu::detail::error_from(st);
+    return y;
+}
+
 }  // namespace scir_signal
+
+namespace scir_gpu {
+
+// For L = sum(g * fir1d_batched_f32(x, taps)): (dL/dx, dL/dtaps), taps in
+// the reference's order, on the HIP device.  No counterpart in the reference.
+inline std::pair<Array2<float>, std::vector<float>> fir1d_batched_f32_vjp_hip(
+    const ArrayView2<float> &x, const std::vector<float> &taps, const ArrayView2<float> &g,
+    scir_hip_precision precision = SCIR_HIP_PREC_F32)
+{
+    if (g.rows != x.rows || g.cols != x.cols)
+        throw GpuError(GpuError::Kind::ShapeMismatch, "g must have the shape of x");
+    Array2<float> xs = detail::standard(x), gs = detail::standard(g);
+    std::vector<float> b = detail::reversed(taps);
+    Array2<float> dx(x.rows, x.cols);
+    std::vector<float> db(b.size(), 0.0f);
+    const int64_t n = (int64_t)x.cols, ld = n > 0 ? n : 1;
+    int st = scir_hip_fir_vjp_f32_host(xs.data.data(), ld, gs.data.data(), ld, b.data(),
+                                       (int32_t)b.size(), (int64_t)x.rows, n, dx.data.data(), ld,
+                                       db.data(), precision);
+    if (st != SCIR_HIP_OK) throw detail::error_from(st);
+    return {dx, std::vector<float>(db.rbegin(), db.rend())};
+}
+
+}  // namespace scir_gpu

@@ -60,6 +60,33 @@ int main(int argc, char **argv)
         std::printf("hip arm: %s\n", e.what());
     }
     CHECK(threw != want_dev);
+
+    // sosfilt: a one-pole section y[n] = 0.5 x[n] + 0.5 y[n-1] (the CPU arm is
+    // the reference loop); the HIP arm gives the same bits (reference order
+    // for rows this short) or, without a device, throws.
+    const scir_signal::Sos sos = {{0.5, 0.0, 0.0, 1.0, -0.5, 0.0}};
+    const std::vector<double> imp = {1.0, 0.0, 0.0, 0.0};
+    std::vector<double> ys = scir_signal::sosfilt(sos, imp);
+    CHECK(ys.size() == 4 && ys[0] == 0.5 && ys[1] == 0.25 && ys[2] == 0.125 && ys[3] == 0.0625);
+    std::vector<double> yf = scir_signal::filtfilt(sos, imp);
+    CHECK(yf.size() == 4 && yf[0] > 0.0);
+    const std::vector<double> h = {0.25, 0.5, 0.25};
+    std::vector<double> yr = scir_signal::resample_poly({1.0, 2.0, 3.0, 4.0, 5.0, 6.0}, 2, 3, h);
+    CHECK(!yr.empty());
+    bool threw2 = false;
+    try {
+        CHECK(scir_signal::sosfilt(sos, imp, Device::Hip) == ys);
+        CHECK(scir_signal::filtfilt(sos, imp, Device::Hip) == yf);
+        CHECK(scir_signal::resample_poly({1.0, 2.0, 3.0, 4.0, 5.0, 6.0}, 2, 3, h, Device::Hip)
+                  .size() == yr.size());
+        auto gr = scir_gpu::fir1d_batched_f32_vjp_hip(x.view(), taps, x.view());
+        // dL/dtaps[j] = sum_i g[i] x[i-(K-1-j)] with g = x
+        CHECK(gr.second.size() == 3 && std::fabs(gr.second[2] - 31.5f) < 1e-4f);
+    } catch (const scir_gpu::GpuError &e) {
+        threw2 = true;
+        CHECK(e.kind() == scir_gpu::GpuError::Kind::BackendUnavailable);
+    }
+    CHECK(threw2 != want_dev);
     if (failures)
         std::printf("host mirror: %d failures\n", failures);
     else
