@@ -99,23 +99,24 @@ def load_traffic(tag):
 
 
 def timed(fn, steps, warmup, stream):
-    """Run fn warmup times, then time `steps` calls with HIP events on
-    `stream`.  Returns (ms_per_step, per-call ms list, wall seconds)."""
+    """Run fn warmup times, then time `steps` back-to-back calls with one pair
+    of HIP events on `stream` around all of them (no event packets between
+    the launches).  Returns (ms_per_step, wall seconds); ms_per_step is the
+    average launch duration including the gaps between launches."""
     import torch
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
     t_wall = time.perf_counter()
-    for i in range(steps):
-        starts[i].record(stream)
+    start.record(stream)
+    for _ in range(steps):
         fn()
-        ends[i].record(stream)
+    end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_wall
-    per = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-    return starts[0].elapsed_time(ends[-1]) / steps, per, wall
+    return start.elapsed_time(end) / steps, wall
 
 
 def cpu_baseline(rows_np, b, budget_s):
@@ -241,7 +242,7 @@ def run_dense(args):
     B, N, K = wl["B"], wl["N"], wl["K"]
     y = torch.empty_like(x)
     stream = torch.cuda.current_stream()
-    ms, per, wall = timed(lambda: device.fir(x, b, prec, out=y), args.steps, args.warmup, stream)
+    ms, wall = timed(lambda: device.fir(x, b, prec, out=y), args.steps, args.warmup, stream)
     value = B * N / (ms * 1e-3) / 1e9
     rows = [0, B // 2, B - 1]
     out = dict(metric=METRIC, value=round(value, 2), unit="Gsamples/s", n_gpus=1,
@@ -253,13 +254,13 @@ def run_dense(args):
                            outliers="1% of samples x1e3" if wl["outliers"] else "none",
                            precision=prec),
                rel_l2_vs_fp64=rel_l2_rows(x, y, b_np, rows), wall_s=round(wall, 3),
-               roofline=roofline_for(wl, name, prec, float(np.mean(per)) * 1e-3))
+               roofline=roofline_for(wl, name, prec, ms * 1e-3))
     if not args.quick:
         comp = {}
         for other in ("f32", "fp16", "bf16x3", "fft32"):
             if other == prec or (other == "fft32" and K > 8192):
                 continue
-            ms_o, _, _ = timed(lambda: device.fir(x, b, other, out=y), max(3, args.steps // 4),
+            ms_o, _ = timed(lambda: device.fir(x, b, other, out=y), max(3, args.steps // 4),
                                1, stream)
             comp["fp32_direct" if other == "f32" else other] = dict(
                 gsamples_s=round(B * N / (ms_o * 1e-3) / 1e9, 2), ms_per_step=round(ms_o, 4),
@@ -269,8 +270,8 @@ def run_dense(args):
             wl2, x2, b2, _ = dense_setup("direct_k33")
             y2 = torch.empty_like(x2)
             # 60 untimed launches (~26 ms): the clock settles after a load step.
-            ms2, per2, _ = timed(lambda: device.fir(x2, b2, "f32", out=y2), 20, 60, stream)
-            r2 = roofline_for(wl2, "direct_k33", "f32", float(np.mean(per2)) * 1e-3)
+            ms2, _ = timed(lambda: device.fir(x2, b2, "f32", out=y2), 20, 60, stream)
+            r2 = roofline_for(wl2, "direct_k33", "f32", ms2 * 1e-3)
             out["bandwidth_workload"] = dict(
                 workload="direct_k33", rows=wl2["B"], samples_per_row=wl2["N"], taps=33,
                 precision="f32", gsamples_s=round(wl2["B"] * wl2["N"] / (ms2 * 1e-3) / 1e9, 2),
@@ -287,6 +288,37 @@ def run_dense(args):
     return out
 
 
+def ragged_rel_l2(x, y, offs, boffs, taps, n=8):
+    """Relative L2 error vs the FP64 oracle over a seeded subset of signals
+    (the shortest, the longest and n - 2 drawn at random)."""
+    from oracle import oracle as orc
+    lens = np.diff(offs)
+    r = np.random.default_rng(0x5C12)
+    pick = sorted(set([int(np.argmin(lens)), int(np.argmax(lens))] +
+                      [int(v) for v in r.integers(0, lens.size, n - 2)]))
+    ys, refs = [], []
+    for s in pick:
+        seg = slice(int(offs[s]), int(offs[s + 1]))
+        ys.append(y[seg].cpu().numpy())
+        refs.append(orc.lfilter_f64(x[seg].cpu().numpy(), taps[boffs[s]:boffs[s + 1]])[0])
+    return orc.rel_l2(np.concatenate(ys), np.concatenate(refs)), len(pick)
+
+
+def ragged_roofline(prec, macs, issued, ms):
+    ach = 2.0 * macs / (ms * 1e-3) / 1e12
+    if prec in ("bf16x3", "fp16"):
+        prods = 6 if prec == "bf16x3" else 1
+        iss = prods * issued / (ms * 1e-3) / 1e12
+        return dict(bound="mfma", achieved=round(ach, 2), peak=BF16_PEAK_TFLOPS,
+                    unit="TFLOP/s", frac=round(ach / BF16_PEAK_TFLOPS, 4),
+                    traffic=load_traffic(f"fir_ragged_{prec}_ragged_pmc"),
+                    mfma_issued_tflops=round(iss, 1),
+                    mfma_issued_frac=round(iss / BF16_PEAK_TFLOPS, 4))
+    return dict(bound="fp32", achieved=round(ach, 2), peak=FP32_PEAK_TFLOPS,
+                unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 4),
+                traffic=load_traffic(f"fir_ragged_{prec}_ragged_pmc"))
+
+
 def run_ragged(args):
     import torch
     from scir_amd import configs, device
@@ -296,26 +328,31 @@ def run_ragged(args):
     x = device.synth_normal(1, total, seed=configs.SEED_BASE + 3).view(-1)
     bt = torch.tensor(taps, device="cuda")
     y = torch.empty_like(x)
-    plan = RaggedPlan(offs, boffs, "f32")
+    prec = args.precision or "f32"
+    if prec not in ("f32", "f32_exact", "bf16x3", "fp16"):
+        raise SystemExit("--precision for ragged: f32, f32_exact, bf16x3 or fp16")
+    plan = RaggedPlan(offs, boffs, prec)
     stream = torch.cuda.current_stream()
-    ms, per, wall = timed(lambda: plan.execute(x, bt, y), args.steps, args.warmup, stream)
-    macs = float(np.sum(np.diff(offs) * np.diff(boffs)))
+    ms, wall = timed(lambda: plan.execute(x, bt, y), args.steps, args.warmup, stream)
+    lens, ks = np.diff(offs), np.diff(boffs)
+    macs = float(np.sum(lens * ks))
+    # MFMA flops issued: 32x32x16 blocks of 8192-output segments, per product.
+    segs = (lens + 8191) // 8192
+    issued = float(np.sum(segs * 8192 * 32.0 * np.array([mfma_steps(int(k)) for k in ks])))
+    err, nsig = ragged_rel_l2(x, y, offs, boffs, taps)
     out = dict(metric=METRIC, value=round(total / (ms * 1e-3) / 1e9, 2), unit="Gsamples/s",
                n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 4),
-               higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
+               higher_is_better=True, scaling="weak", vs_baseline=None, dtype=prec,
                data="synthetic",
                config=dict(workload="ragged", signals=offs.size - 1, samples=total,
                            lengths="log-uniform [256, 2^20]", taps="uniform [15, 1023]",
-                           precision="f32"),
-               wall_s=round(wall, 3))
-    ach = 2.0 * macs / (float(np.mean(per)) * 1e-3) / 1e12
-    out["roofline"] = dict(bound="fp32", achieved=round(ach, 2), peak=FP32_PEAK_TFLOPS,
-                           unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 4),
-                           traffic=load_traffic("fir_ragged_f32_ragged_pmc"))
+                           precision=prec),
+               rel_l2_vs_fp64=err, rel_l2_signals=nsig, wall_s=round(wall, 3),
+               roofline=ragged_roofline(prec, macs, issued, ms))
     if args.quick:
         plan.close()
         return out
-    ms_s, _, _ = timed(lambda: plan.execute_per_signal(x, bt, y), 1, 1, stream)
+    ms_s, _ = timed(lambda: plan.execute_per_signal(x, bt, y), 1, 1, stream)
     side = torch.cuda.Stream()
     torch.cuda.synchronize()
     with Graph(side) as g:
@@ -323,11 +360,21 @@ def run_ragged(args):
 
     def replay():
         g.launch()
-    ms_g, _, _ = timed(replay, max(3, args.steps // 2), 1, side)
+    ms_g, _ = timed(replay, max(3, args.steps // 2), 1, side)
     out["compare"] = dict(per_signal_launches_ms=round(ms_s, 3), graph_replay_ms=round(ms_g, 3),
                           persistent_speedup=round(ms_s / ms, 2))
     g.close()
     plan.close()
+    for other in ("f32", "bf16x3"):
+        if other == prec:
+            continue
+        p2 = RaggedPlan(offs, boffs, other)
+        ms_o, _ = timed(lambda: p2.execute(x, bt, y), max(3, args.steps // 2), 2, stream)
+        e2, _ = ragged_rel_l2(x, y, offs, boffs, taps)
+        out["compare"][other] = dict(gsamples_s=round(total / (ms_o * 1e-3) / 1e9, 2),
+                                     ms_per_step=round(ms_o, 4), rel_l2_vs_fp64=e2,
+                                     roofline=ragged_roofline(other, macs, issued, ms_o))
+        p2.close()
     return out
 
 
@@ -347,10 +394,10 @@ def run_vjp(args):
         device.vjp_dx(w, b, prec, out=dx)
         device.vjp_db(w, x, K, out=db)
     stream = torch.cuda.current_stream()
-    ms, per, wall = timed(step, args.steps, args.warmup, stream)
-    ms_db, per_db, _ = timed(lambda: device.vjp_db(w, x, K, out=db), 5, 1, stream)
+    ms, wall = timed(step, args.steps, args.warmup, stream)
+    ms_db, _ = timed(lambda: device.vjp_db(w, x, K, out=db), 5, 1, stream)
     flops_db = 2.0 * K * B * N
-    ach = flops_db / (float(np.mean(per_db)) * 1e-3) / 1e12
+    ach = flops_db / (ms_db * 1e-3) / 1e12
     return dict(metric=METRIC, value=round(B * N / (ms * 1e-3) / 1e9, 2), unit="Gsamples/s",
                 n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 4),
                 higher_is_better=True, scaling="weak", vs_baseline=None, dtype=prec,
@@ -384,9 +431,9 @@ def run_resample(args):
     def step():
         _lib.check(L.scir_hip_resample_poly_f64_dev(vp(x), N, B, N, up, down, vp(hd), h.size,
                                                      vp(y), M, ctypes.c_void_p(stream.cuda_stream)))
-    ms, per, wall = timed(step, args.steps, args.warmup, stream)
+    ms, wall = timed(step, args.steps, args.warmup, stream)
     alg = 8.0 * B * (N + M)
-    gbs = alg / (float(np.mean(per)) * 1e-3) / 1e9
+    gbs = alg / (ms * 1e-3) / 1e9
     out = dict(metric="polyphase resample Gsamples/s (output samples)",
                value=round(B * M / (ms * 1e-3) / 1e9, 2), unit="Gsamples/s", n_gpus=1,
                steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 4),
@@ -450,10 +497,10 @@ def run_sosfilt(args):
     def step():
         _lib.check(fn(vp(x), N, B, N, vp(sos), sos_np.shape[0], vp(y), N, flags,
                       ctypes.c_void_p(stream.cuda_stream)))
-    ms, per, wall = timed(step, args.steps, args.warmup, stream)
+    ms, wall = timed(step, args.steps, args.warmup, stream)
     passes = 2 if mode == "filtfilt" else 1
     alg = 16.0 * B * N * passes
-    gbs = alg / (float(np.mean(per)) * 1e-3) / 1e9
+    gbs = alg / (ms * 1e-3) / 1e9
     tag = {"chunked": "sosfilt", "reference_order": "sosfilt_refo", "filtfilt": "filtfilt"}[mode]
     out = dict(metric="sosfilt Gsamples/s", value=round(B * N / (ms * 1e-3) / 1e9, 2),
                unit="Gsamples/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
@@ -498,8 +545,8 @@ def run_copy(args):
     x = device.synth_normal(1, n, seed=1).view(-1)
     y = torch.empty_like(x)
     stream = torch.cuda.current_stream()
-    ms, per, wall = timed(lambda: y.copy_(x), args.steps, args.warmup, stream)
-    gbs = 8.0 * n / (float(np.mean(per)) * 1e-3) / 1e9
+    ms, wall = timed(lambda: y.copy_(x), args.steps, args.warmup, stream)
+    gbs = 8.0 * n / (ms * 1e-3) / 1e9
     return dict(metric="device copy GB/s (calibration)", value=round(gbs, 1), unit="GB/s",
                 n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 4),
                 higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",

@@ -221,7 +221,11 @@ SCIR_HIP_API int scir_hip_filtfilt_f64_cpu(const double *x, int64_t ldx, int64_t
  * is x[offs[s] .. offs[s+1]) with taps b[b_offs[s] .. b_offs[s+1]); y has the
  * same layout as x.  A plan holds the host-built, cost-sorted work list and
  * the device work queue; executing it is one persistent launch.  Executing a
- * plan does not allocate, so it may be captured in a HIP graph. */
+ * plan does not allocate, so it may be captured in a HIP graph.
+ * precision: F32 / AUTO (FP32 direct form), F32_EXACT (the reference loop's
+ * order, bitwise), or BF16X3 / FP16 (the MFMA Toeplitz contraction with the
+ * dense MFMA kernel's arithmetic, segments of 8192 outputs; taps up to the LDS
+ * limit: K <= 5601 for BF16X3).  FFT32 gives SCIR_HIP_ERR_UNSUPPORTED. */
 typedef struct scir_hip_ragged_plan_s *scir_hip_ragged_plan;
 /* offs[S+1], b_offs[S+1]: HOST arrays (int64, non-decreasing, offs[0] ==
  * b_offs[0] == 0). */

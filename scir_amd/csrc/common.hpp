@@ -101,6 +101,24 @@ int fir_dense_dispatch(const float *x, int64_t ldx, const float *b, int32_t K, i
                        hipStream_t stream, DeviceState *st);
 int fill_zero_2d(float *y, int64_t ldy, int64_t B, int64_t N, hipStream_t stream);
 
+// ---- ragged (CSR) batches on MFMA ----------------------------------------
+// One work item: segments seg0 .. seg0+nseg-1 (8192 outputs each) of the
+// signal x[x0 .. x0+len) with taps b[b0 .. b0+K) (lfilter order).
+struct MfmaRaggedItem {
+    int64_t x0, len, b0;
+    int32_t K, seg0, nseg, pad;
+};
+constexpr int kMfmaRaggedSegLen = 8192;  // outputs per segment (4 waves x 2 tiles x 32 x 32)
+// Dynamic LDS bytes of the ragged MFMA kernel for taps up to kmax; fails with
+// SCIR_HIP_ERR_UNSUPPORTED when that exceeds the CU's LDS.
+int fir_mfma_ragged_lds(int precision, int kmax, size_t *lds);
+// Resident workgroups per CU of the ragged MFMA kernel at `lds` bytes.
+int fir_mfma_ragged_occupancy(int precision, size_t lds, int *per_cu);
+// Persistent launch over `items` (pulled through `counter`, which must read 0).
+int fir_mfma_ragged_launch(const float *x, const float *b, float *y, const MfmaRaggedItem *items,
+                           int n_items, unsigned int *counter, int precision, int kmax,
+                           size_t lds, int grid, hipStream_t stream);
+
 // Autotuned choices (defaults until a table is loaded or a search ran).
 struct TuneChoice {
     int direct_r = 8;         // outputs per lane in the direct kernel

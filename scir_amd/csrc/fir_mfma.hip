@@ -61,7 +61,7 @@ constexpr int kMaxLds = 160 * 1024;
 // is longer than the prefetched part (2*m_hi > 256 units, K > 2049).
 constexpr int kRestBatch = 2;
 
-inline int mfma_m_hi(int K) { return (K + 14) / 16; }  // ceil((K-1)/16) for K >= 1
+__host__ __device__ inline int mfma_m_hi(int K) { return (K + 14) / 16; }  // ceil((K-1)/16), K >= 1
 inline int mfma_steps(int K) { return mfma_m_hi(K) + 2; }
 
 // Byte address of LDS unit u in a split array whose lanes read 4*TW apart.
@@ -305,6 +305,120 @@ __device__ __forceinline__ void store_unit(unsigned char *lds, int split_stride,
     }
 }
 
+// Tap tables for taps b[0, K) (lfilter order): T_s[j] = split_s(b[C - j]),
+// j in [0, L), in two copies one entry apart (copy c holds T_s[pos - c] at
+// entry pos), so every lane's 16-byte A read starts 4-byte aligned in one of
+// them.  Zero outside the taps.
+template <int S, int TW>
+__device__ __forceinline__ void build_tap_table(unsigned char *lds, int tab_offset,
+                                                int copy_bytes, const float *__restrict__ b,
+                                                int K, int m_hi)
+{
+    const int C = 16 * (m_hi + 2 * TW + 1) + 32, L = C + 32 * TW + 32;
+    for (int q = threadIdx.x; q < 2 * (L + 1); q += kThreads) {
+        const int c = q / (L + 1), pos = q - c * (L + 1), j = pos - c;
+        const int k = C - j;
+        const float v = (j >= 0 && j < L && k >= 0 && k < K) ? b[k] : 0.0f;
+        unsigned short t[3];
+        if (S == 3)
+            split3(v, t[0], t[1], t[2]);
+        else
+            t[0] = f16_bits((_Float16)v);
+#pragma unroll
+        for (int s2 = 0; s2 < S; ++s2)
+            *reinterpret_cast<unsigned short *>(lds + tab_offset + (2 * s2 + c) * copy_bytes +
+                                                2 * pos) = t[s2];
+    }
+}
+
+// Stage LDS units 0 .. units-1 of a segment (unit 0 = sample xb of the row):
+// the first PF units of each lane come from `raw` (loaded ahead), the rest
+// are loaded here.
+template <int S, int TW, int PF>
+__device__ __forceinline__ void stage_segment(unsigned char *lds, int split_stride,
+                                              const f32x4 (&raw)[PF][2],
+                                              const float *__restrict__ xrow, int64_t N,
+                                              int64_t xb, int units, int vec_x, int rev, int tid)
+{
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+        const int u = tid + j * kThreads;
+        if (u < units) store_unit<S, TW>(lds, split_stride, u, raw[j]);
+    }
+    for (int u0 = tid + PF * kThreads; u0 < units; u0 += kRestBatch * kThreads) {
+        f32x4 rest[kRestBatch][2];
+#pragma unroll
+        for (int j = 0; j < kRestBatch; ++j)
+            load_unit(xrow, N, xb, u0 + j * kThreads, units, vec_x, rev, rest[j]);
+#pragma unroll
+        for (int j = 0; j < kRestBatch; ++j) {
+            const int u = u0 + j * kThreads;
+            if (u >= units) break;
+            store_unit<S, TW>(lds, split_stride, u, rest[j]);
+        }
+    }
+}
+
+// Contraction of one staged segment by wave `wave`; lane (i, g) = (lane % 32,
+// lane / 32).  The caller's barrier has made the staged units and the tap
+// tables visible.
+template <int S, int TW>
+__device__ __forceinline__ void contract_segment(Contraction<S, TW> &ct, const unsigned char *lds,
+                                                 int split_stride, int tab_offset,
+                                                 int copy_bytes, int msteps, int m_hi, int wave,
+                                                 int i, int g)
+{
+    const int C = 16 * (m_hi + 2 * TW + 1) + 32;
+    ct.lds = lds;
+    ct.split_stride = split_stride;
+    ct.msteps = msteps;
+    ct.Q = msteps + 2 * TW - 2;
+    // D(n) = tile 0's B at half-step q = n - 2TW + 1: unit
+    // 4*(wave offset + TW*i) + 2*(m_hi - q) + g.
+    ct.ubase = 4 * (wave * 32 * TW + TW * i) + 2 * (m_hi + 2 * TW - 1) + g;
+    {
+        // A of half-step index sa (m = sa - 1) starts at entry
+        // j0 = C - 16*(sa - 1) - i + 8g, in the copy whose shift c makes j0 + c even.
+        const int c = (C - i) & 1;
+#pragma unroll
+        for (int s2 = 0; s2 < S; ++s2)
+            ct.abase[s2] = tab_offset + (2 * s2 + c) * copy_bytes + 2 * (C + 16 - i + 8 * g + c);
+    }
+    ct.run();
+}
+
+// Epilogue: tile t, lane (i, g) holds block P0 + wave*32*TW + t + TW*i,
+// samples 8q + 4g + j; outputs past N are dropped.
+template <int S, int TW>
+__device__ __forceinline__ void store_segment(const Contraction<S, TW> &ct,
+                                              float *__restrict__ yrow, int64_t N, int64_t P0,
+                                              int wave, int i, int g, int vec_y, int rev)
+{
+#pragma unroll
+    for (int t = 0; t < TW; ++t) {
+        const int64_t n0 = 32 * (P0 + wave * 32 * TW + t + TW * i) + 4 * g;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            f32x4 v;
+            v.x = ct.acc0[t][4 * q + 0] + ct.acc1[t][4 * q + 0];
+            v.y = ct.acc0[t][4 * q + 1] + ct.acc1[t][4 * q + 1];
+            v.z = ct.acc0[t][4 * q + 2] + ct.acc1[t][4 * q + 2];
+            v.w = ct.acc0[t][4 * q + 3] + ct.acc1[t][4 * q + 3];
+            const int64_t n = n0 + 8 * q;
+            if (vec_y && n + 3 < N) {
+                if (!rev)
+                    *reinterpret_cast<f32x4 *>(yrow + n) = v;
+                else
+                    *reinterpret_cast<f32x4 *>(yrow + (N - 4 - n)) = f32x4{v.w, v.z, v.y, v.x};
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (n + j < N) yrow[rev ? N - 1 - (n + j) : n + j] = v[j];
+            }
+        }
+    }
+}
+
 // One workgroup (4 waves) computes segs_per_wg consecutive segments of
 // kWaves*TW*32 output blocks each (segment id = row * segs_per_row + index in
 // the row).  A segment needs LDS units 0 .. units-1: 2*m_hi units of history
@@ -347,27 +461,9 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
     };
     prefetch(seg_first, (int)threadIdx.x);
 
-    // ---- tap tables: T_s[j] = split_s(b[C - j]), j in [0, L), two copies ----
-    // (built while the first segment's loads are in flight)
-    {
-        const int C = 16 * (m_hi + 2 * TW + 1) + 32, L = C + 32 * TW + 32;
-        for (int q = threadIdx.x; q < 2 * (L + 1); q += kThreads) {
-            const int c = q / (L + 1), pos = q - c * (L + 1), j = pos - c;
-            const int k = C - j;
-            const float v = (j >= 0 && j < L && k >= 0 && k < K) ? b[k] : 0.0f;
-            unsigned short t[3];
-            if (S == 3)
-                split3(v, t[0], t[1], t[2]);
-            else
-                t[0] = f16_bits((_Float16)v);
-#pragma unroll
-            for (int s2 = 0; s2 < S; ++s2)
-                *reinterpret_cast<unsigned short *>(lds + tab_offset + (2 * s2 + c) * copy_bytes +
-                                                    2 * pos) = t[s2];
-        }
-    }
+    // Tap tables, built while the first segment's loads are in flight.
+    build_tap_table<S, TW>(lds, tab_offset, copy_bytes, b, K, m_hi);
 
-    const int C = 16 * (m_hi + 2 * TW + 1) + 32;
     for (int64_t sg = seg_first; sg < seg_end; ++sg) {
         // Lane-dependent values are recomputed in every iteration rather than
         // hoisted out of the loop, where they would hold registers through
@@ -382,72 +478,86 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
 
         // ---- stage: x[xb .. xb + 8*units) -> S split arrays in LDS ----
         if (sg != seg_first) __syncthreads();  // the previous segment's MFMAs are done
-#pragma unroll
-        for (int j = 0; j < PF; ++j) {
-            const int u = tid + j * kThreads;
-            if (u < units) store_unit<S, TW>(lds, split_stride, u, raw[j]);
-        }
-        for (int u0 = tid + PF * kThreads; u0 < units; u0 += kRestBatch * kThreads) {
-            f32x4 rest[kRestBatch][2];
-#pragma unroll
-            for (int j = 0; j < kRestBatch; ++j)
-                load_unit(xrow, N, xb, u0 + j * kThreads, units, vec_x, rev, rest[j]);
-#pragma unroll
-            for (int j = 0; j < kRestBatch; ++j) {
-                const int u = u0 + j * kThreads;
-                if (u >= units) break;
-                store_unit<S, TW>(lds, split_stride, u, rest[j]);
-            }
-        }
+        stage_segment<S, TW, PF>(lds, split_stride, raw, xrow, N, xb, units, vec_x, rev, tid);
         __syncthreads();
         if (sg + 1 < seg_end) prefetch(sg + 1, tid);
 
-        // ---- contraction ----
+        // ---- contraction and epilogue ----
         Contraction<S, TW> ct;
-        ct.lds = lds;
-        ct.split_stride = split_stride;
-        ct.msteps = msteps;
-        ct.Q = msteps + 2 * TW - 2;
-        // D(n) = tile 0's B at half-step q = n - 2TW + 1: unit
-        // 4*(wave offset + TW*i) + 2*(m_hi - q) + g.
-        ct.ubase = 4 * (wave * 32 * TW + TW * i) + 2 * (m_hi + 2 * TW - 1) + g;
-        {
-            // A of half-step index sa (m = sa - 1) starts at entry
-            // j0 = C - 16*(sa - 1) - i + 8g, in the copy whose shift c makes j0 + c even.
-            const int c = (C - i) & 1;
-#pragma unroll
-            for (int s2 = 0; s2 < S; ++s2)
-                ct.abase[s2] = tab_offset + (2 * s2 + c) * copy_bytes + 2 * (C + 16 - i + 8 * g + c);
-        }
-        ct.run();
+        contract_segment<S, TW>(ct, lds, split_stride, tab_offset, copy_bytes, msteps, m_hi, wave,
+                                i, g);
+        store_segment<S, TW>(ct, y + row * ldy, N, P0, wave, i, g, vec_y, rev);
+    }
+}
 
-        // ---- epilogue: tile t, lane (i, g): block P + t + TW*i, sample 8q + 4g + j ----
-        float *__restrict__ yrow = y + row * ldy;
+// Ragged (CSR) batches: a persistent grid pulls work items (a run of
+// consecutive segments of one signal, longest first) through an atomic
+// counter until the queue is drained; every workgroup leaves the loop on its
+// first fetch past the end.  Each segment is staged, contracted and stored as
+// in the dense kernel, with the item's own taps and length; the tap tables are
+// rebuilt only when the item's taps differ from the ones in LDS.  The item
+// index is broadcast through the 16 bytes of LDS at `slot_offset`.
+template <int S, int TW>
+__global__ __launch_bounds__(kThreads) void fir_mfma_ragged_kernel(
+    const float *__restrict__ x, const float *__restrict__ b, float *__restrict__ y,
+    const MfmaRaggedItem *__restrict__ items, int n_items, unsigned int *__restrict__ counter,
+    int split_stride, int tab_offset, int copy_bytes, int slot_offset)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    constexpr int SEG_BLOCKS = kWaves * TW * 32;
+    constexpr int PF = 2 * TW + 1;
+    int *s_item = reinterpret_cast<int *>(lds + slot_offset);
+    int64_t tab_b0 = -1;
+    int tab_K = -1;
+    f32x4 raw[PF][2];
+    for (;;) {
+        if (threadIdx.x == 0) *s_item = (int)atomicAdd(counter, 1u);
+        __syncthreads();  // every wave is also done with the previous item's LDS
+        const int it = *s_item;
+        __syncthreads();  // every lane has read the slot before it is written again
+        if (it >= n_items) break;
+        const MfmaRaggedItem w = items[it];
+        const int m_hi = mfma_m_hi(w.K), msteps = m_hi + 2;
+        const int units = SEG_BLOCKS * 4 + 2 * m_hi;
+        const float *__restrict__ xrow = x + w.x0;
+        float *__restrict__ yrow = y + w.x0;
+        const int vec_x = (reinterpret_cast<uintptr_t>(xrow) & 15) == 0;
+        const int vec_y = (reinterpret_cast<uintptr_t>(yrow) & 15) == 0;
+        const int64_t seg_end = (int64_t)w.seg0 + w.nseg;
+        auto prefetch = [&](int64_t sg, int tid) {
+            const int64_t xb = 32 * sg * SEG_BLOCKS - 16 * (int64_t)m_hi;
 #pragma unroll
-        for (int t = 0; t < TW; ++t) {
-            const int64_t n0 = 32 * (P0 + wave * 32 * TW + t + TW * i) + 4 * g;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                f32x4 v;
-                v.x = ct.acc0[t][4 * q + 0] + ct.acc1[t][4 * q + 0];
-                v.y = ct.acc0[t][4 * q + 1] + ct.acc1[t][4 * q + 1];
-                v.z = ct.acc0[t][4 * q + 2] + ct.acc1[t][4 * q + 2];
-                v.w = ct.acc0[t][4 * q + 3] + ct.acc1[t][4 * q + 3];
-                const int64_t n = n0 + 8 * q;
-                if (vec_y && n + 3 < N) {
-                    if (!rev)
-                        *reinterpret_cast<f32x4 *>(yrow + n) = v;
-                    else
-                        *reinterpret_cast<f32x4 *>(yrow + (N - 4 - n)) = f32x4{v.w, v.z, v.y, v.x};
-                } else {
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        if (n + j < N) yrow[rev ? N - 1 - (n + j) : n + j] = v[j];
-                }
-            }
+            for (int j = 0; j < PF; ++j)
+                load_unit(xrow, w.len, xb, tid + j * kThreads, units, vec_x, 0, raw[j]);
+        };
+        prefetch(w.seg0, (int)threadIdx.x);
+        if (w.b0 != tab_b0 || w.K != tab_K) {
+            build_tap_table<S, TW>(lds, tab_offset, copy_bytes, b + w.b0, w.K, m_hi);
+            tab_b0 = w.b0;
+            tab_K = w.K;
+        }
+        for (int64_t sg = w.seg0; sg < seg_end; ++sg) {
+            int tid = (int)threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            const int wave = tid >> 6, lane = tid & 63;
+            const int i = lane & 31, g = lane >> 5;
+            const int64_t P0 = sg * SEG_BLOCKS;
+            const int64_t xb = 32 * P0 - 16 * (int64_t)m_hi;
+            if (sg != w.seg0) __syncthreads();  // the previous segment's MFMAs are done
+            stage_segment<S, TW, PF>(lds, split_stride, raw, xrow, w.len, xb, units, vec_x, 0,
+                                     tid);
+            __syncthreads();
+            if (sg + 1 < seg_end) prefetch(sg + 1, tid);
+            Contraction<S, TW> ct;
+            contract_segment<S, TW>(ct, lds, split_stride, tab_offset, copy_bytes, msteps, m_hi,
+                                    wave, i, g);
+            store_segment<S, TW>(ct, yrow, w.len, P0, wave, i, g, vec_y, 0);
         }
     }
 }
+
+constexpr int kRaggedTW = 2;
+static_assert(kMfmaRaggedSegLen == 32 * kWaves * kRaggedTW * 32, "segment length");
 
 int align_ok(const float *p, int64_t ld)
 {
@@ -539,6 +649,83 @@ int fir_mfma_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, 
         return run_mfma<3>(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
     if (precision == SCIR_HIP_PREC_FP16)
         return run_mfma<1>(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
+    return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "precision %d is not an MFMA mode", precision);
+}
+
+namespace {
+
+// Ragged LDS layout: S split arrays, the tap tables, then the item slot.
+template <int S>
+struct RaggedLayout {
+    int split_stride, tab_offset, copy_bytes, slot_offset;
+    size_t bytes;
+    explicit RaggedLayout(int kmax)
+    {
+        const int m_hi = mfma_m_hi(kmax);
+        const int units = kWaves * kRaggedTW * 32 * 4 + 2 * m_hi;
+        split_stride = (int)unit_bytes(units, kRaggedTW);
+        tab_offset = S * split_stride;
+        copy_bytes = tab_copy_bytes(m_hi, kRaggedTW);
+        slot_offset = tab_offset + 2 * S * copy_bytes;
+        bytes = (size_t)slot_offset + 16;
+    }
+};
+
+template <int S>
+int ragged_launch(const float *x, const float *b, float *y, const MfmaRaggedItem *items,
+                  int n_items, unsigned int *counter, int kmax, size_t lds, int grid,
+                  hipStream_t stream)
+{
+    const RaggedLayout<S> lay(kmax);
+    if (lay.bytes != lds || lds > (size_t)kMaxLds)
+        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "ragged MFMA layout mismatch");
+    static bool attr_set = false;  // guarded by the device mutex held by the caller
+    if (!attr_set) {
+        SCIR_HIP_TRY(hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&fir_mfma_ragged_kernel<S, kRaggedTW>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((fir_mfma_ragged_kernel<S, kRaggedTW>), dim3((unsigned)grid),
+                       dim3(kThreads), lds, stream, x, b, y, items, n_items, counter,
+                       lay.split_stride, lay.tab_offset, lay.copy_bytes, lay.slot_offset);
+    return check_launch("fir_mfma_ragged_kernel");
+}
+
+}  // namespace
+
+int fir_mfma_ragged_lds(int precision, int kmax, size_t *lds)
+{
+    const size_t bytes = precision == SCIR_HIP_PREC_BF16X3 ? RaggedLayout<3>(kmax).bytes
+                                                           : RaggedLayout<1>(kmax).bytes;
+    if (bytes > (size_t)kMaxLds)
+        return fail(SCIR_HIP_ERR_UNSUPPORTED,
+                    "ragged MFMA batches take taps up to the LDS limit; K=%d needs %zu bytes",
+                    kmax, bytes);
+    *lds = bytes;
+    return SCIR_HIP_OK;
+}
+
+int fir_mfma_ragged_occupancy(int precision, size_t lds, int *per_cu)
+{
+    int n = 0;
+    const void *fn = precision == SCIR_HIP_PREC_BF16X3
+                         ? reinterpret_cast<const void *>(&fir_mfma_ragged_kernel<3, kRaggedTW>)
+                         : reinterpret_cast<const void *>(&fir_mfma_ragged_kernel<1, kRaggedTW>);
+    SCIR_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds));
+    SCIR_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kThreads, lds));
+    *per_cu = std::max(1, n);
+    return SCIR_HIP_OK;
+}
+
+int fir_mfma_ragged_launch(const float *x, const float *b, float *y, const MfmaRaggedItem *items,
+                           int n_items, unsigned int *counter, int precision, int kmax,
+                           size_t lds, int grid, hipStream_t stream)
+{
+    if (precision == SCIR_HIP_PREC_BF16X3)
+        return ragged_launch<3>(x, b, y, items, n_items, counter, kmax, lds, grid, stream);
+    if (precision == SCIR_HIP_PREC_FP16)
+        return ragged_launch<1>(x, b, y, items, n_items, counter, kmax, lds, grid, stream);
     return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "precision %d is not an MFMA mode", precision);
 }
 

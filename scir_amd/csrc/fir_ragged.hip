@@ -18,6 +18,9 @@
 // workgroup leaves the loop on its first fetch past the end.  Items are
 // filtered with the direct-form tile code of the dense kernels (same
 // arithmetic, so the EXACT mode is bit-identical to the reference loop).
+// With an MFMA precision (BF16x3 or FP16) the items are runs of 8192-output
+// segments and the persistent kernel is the MFMA Toeplitz contraction of
+// fir_mfma.hip, with the same split arithmetic as the dense MFMA kernel.
 #include "direct_tiles.hpp"
 
 #include <algorithm>
@@ -92,9 +95,11 @@ struct scir_hip_ragged_plan_s {
     int64_t S = 0;
     std::vector<int64_t> offs, boffs;  // host copies (per-signal launches)
     int n_items = 0;
-    RaggedItem *items = nullptr;       // device
+    RaggedItem *items = nullptr;       // device (direct form)
+    MfmaRaggedItem *mitems = nullptr;  // device (MFMA)
     unsigned int *counter = nullptr;   // device
     int grid = 0;
+    int kmax = 0;
     size_t lds = 0;
 };
 
@@ -117,20 +122,27 @@ int validate_csr(const int64_t *offs, const int64_t *b_offs, int64_t S)
     return SCIR_HIP_OK;
 }
 
-int ragged_precision(int32_t p, bool *exact)
+enum RaggedMode { kDirect, kDirectExact, kMfma };
+
+// Kernel family of a precision: the FP32 direct form (f32 / auto, or the
+// reference order with f32_exact) or the MFMA contraction (bf16x3 / fp16).
+int ragged_mode(int32_t p, RaggedMode *mode)
 {
     switch (p) {
     case SCIR_HIP_PREC_F32:
-    case SCIR_HIP_PREC_AUTO: *exact = false; return SCIR_HIP_OK;
-    case SCIR_HIP_PREC_F32_EXACT: *exact = true; return SCIR_HIP_OK;
+    case SCIR_HIP_PREC_AUTO: *mode = kDirect; return SCIR_HIP_OK;
+    case SCIR_HIP_PREC_F32_EXACT: *mode = kDirectExact; return SCIR_HIP_OK;
     case SCIR_HIP_PREC_BF16X3:
-    case SCIR_HIP_PREC_FP16:
+    case SCIR_HIP_PREC_FP16: *mode = kMfma; return SCIR_HIP_OK;
     case SCIR_HIP_PREC_FFT32:
         return fail(SCIR_HIP_ERR_UNSUPPORTED,
-                    "ragged batches run the FP32 direct form (f32 / f32_exact / auto)");
+                    "ragged batches run the direct form (f32 / f32_exact / auto) or the MFMA "
+                    "contraction (bf16x3 / fp16)");
     default: return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "unknown precision %d", p);
     }
 }
+
+constexpr int kMfmaSegsPerItem = 8;  // segments of 8192 outputs per MFMA work item, at most
 
 }  // namespace
 
@@ -143,12 +155,14 @@ int scir_hip_ragged_plan_create(const int64_t *offs, const int64_t *b_offs, int6
     if (!plan) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "plan is null");
     *plan = nullptr;
     SCIR_TRY(validate_csr(offs, b_offs, S));
-    bool exact = false;
-    SCIR_TRY(ragged_precision(precision, &exact));
+    RaggedMode mode = kDirect;
+    SCIR_TRY(ragged_mode(precision, &mode));
     DeviceState *st = nullptr;
     SCIR_TRY(current_device_state(&st));
 
+    // Work items and their costs (multiply-adds, or MFMA half-steps x segments).
     std::vector<RaggedItem> items;
+    std::vector<MfmaRaggedItem> mitems;
     std::vector<int64_t> cost;
     int kmax = 0;
     for (int64_t s = 0; s < S; ++s) {
@@ -156,6 +170,25 @@ int scir_hip_ragged_plan_create(const int64_t *offs, const int64_t *b_offs, int6
         const int K = (int)(b_offs[s + 1] - b_offs[s]);
         if (len == 0) continue;
         kmax = std::max(kmax, K);
+        if (mode == kMfma) {
+            const int64_t segs = (len + kMfmaRaggedSegLen - 1) / kMfmaRaggedSegLen;
+            if (segs > 0x7fffffff)
+                return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "signal %lld is too long",
+                            (long long)s);
+            const int64_t per_seg = (K + 14) / 16 + 2 + 2;  // half-steps of the loop
+            for (int64_t g = 0; g < segs; g += kMfmaSegsPerItem) {
+                MfmaRaggedItem it{};
+                it.x0 = offs[s];
+                it.len = len;
+                it.b0 = b_offs[s];
+                it.K = K;
+                it.seg0 = (int32_t)g;
+                it.nseg = (int32_t)std::min<int64_t>(kMfmaSegsPerItem, segs - g);
+                mitems.push_back(it);
+                cost.push_back((int64_t)it.nseg * per_seg);
+            }
+            continue;
+        }
         const int64_t tiles = (len + kT - 1) / kT;
         const int64_t per_tile = (int64_t)kT * std::max(K, 1);
         const int64_t tpi = std::max<int64_t>(1, std::min<int64_t>(tiles, kMinItemWork / per_tile));
@@ -171,13 +204,24 @@ int scir_hip_ragged_plan_create(const int64_t *offs, const int64_t *b_offs, int6
             cost.push_back((int64_t)it.ntiles * per_tile);
         }
     }
-    if (items.size() > 0x7fffffff) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "too many items");
-    std::vector<size_t> order(items.size());
+    const size_t n_items = mode == kMfma ? mitems.size() : items.size();
+    if (n_items > 0x7fffffff) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "too many items");
+    // Longest first (LPT order).
+    std::vector<size_t> order(n_items);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
                      [&](size_t a, size_t c) { return cost[a] > cost[c]; });
-    std::vector<RaggedItem> sorted(items.size());
-    for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+
+    size_t lds = 0;
+    int per_cu = 8;
+    if (mode == kMfma) {
+        SCIR_TRY(fir_mfma_ragged_lds(precision, kmax, &lds));
+        SCIR_TRY(fir_mfma_ragged_occupancy(precision, lds, &per_cu));
+    } else {
+        const int hmax = (std::min(std::max(kmax, 1), kSuper) + 6) & ~3;
+        const int staged = kT + hmax + 4;
+        lds = (size_t)(staged + (staged / 16) * 4 + 4) * sizeof(float);
+    }
 
     auto *p = new scir_hip_ragged_plan_s;
     p->device = st->device;
@@ -185,17 +229,28 @@ int scir_hip_ragged_plan_create(const int64_t *offs, const int64_t *b_offs, int6
     p->S = S;
     p->offs.assign(offs, offs + S + 1);
     p->boffs.assign(b_offs, b_offs + S + 1);
-    p->n_items = (int)sorted.size();
-    const int hmax = (std::min(std::max(kmax, 1), kSuper) + 6) & ~3;
-    const int staged = kT + hmax + 4;
-    p->lds = (size_t)(staged + (staged / 16) * 4 + 4) * sizeof(float);
-    p->grid = std::max(1, std::min(p->n_items, st->cu_count * 8));
+    p->n_items = (int)n_items;
+    p->kmax = kmax;
+    p->lds = lds;
+    p->grid = std::max(1, std::min(p->n_items, st->cu_count * per_cu));
     hipError_t e = hipMalloc(&p->counter, sizeof(unsigned int));
-    if (e == hipSuccess && p->n_items > 0)
-        e = hipMalloc(&p->items, sizeof(RaggedItem) * sorted.size());
-    if (e == hipSuccess && p->n_items > 0)
-        e = hipMemcpy(p->items, sorted.data(), sizeof(RaggedItem) * sorted.size(),
-                      hipMemcpyHostToDevice);
+    if (e == hipSuccess && n_items > 0) {
+        if (mode == kMfma) {
+            std::vector<MfmaRaggedItem> sorted(n_items);
+            for (size_t i = 0; i < n_items; ++i) sorted[i] = mitems[order[i]];
+            e = hipMalloc(&p->mitems, sizeof(MfmaRaggedItem) * n_items);
+            if (e == hipSuccess)
+                e = hipMemcpy(p->mitems, sorted.data(), sizeof(MfmaRaggedItem) * n_items,
+                              hipMemcpyHostToDevice);
+        } else {
+            std::vector<RaggedItem> sorted(n_items);
+            for (size_t i = 0; i < n_items; ++i) sorted[i] = items[order[i]];
+            e = hipMalloc(&p->items, sizeof(RaggedItem) * n_items);
+            if (e == hipSuccess)
+                e = hipMemcpy(p->items, sorted.data(), sizeof(RaggedItem) * n_items,
+                              hipMemcpyHostToDevice);
+        }
+    }
     if (e != hipSuccess) {
         scir_hip_ragged_plan_destroy(p);
         return fail(e == hipErrorOutOfMemory ? SCIR_HIP_ERR_OUT_OF_MEMORY : SCIR_HIP_ERR_BACKEND,
@@ -217,12 +272,19 @@ int scir_hip_ragged_plan_execute(scir_hip_ragged_plan plan, const float *x, cons
     if (dev != plan->device)
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "plan was made on device %d, current is %d",
                     plan->device, dev);
-    bool exact = false;
-    SCIR_TRY(ragged_precision(plan->precision, &exact));
+    RaggedMode mode = kDirect;
+    SCIR_TRY(ragged_mode(plan->precision, &mode));
     hipStream_t s = as_stream(stream);
     hipLaunchKernelGGL(ragged_reset_kernel, dim3(1), dim3(1), 0, s, plan->counter);
     SCIR_TRY(check_launch("ragged_reset_kernel"));
-    if (exact)
+    if (mode == kMfma) {
+        DeviceState *st = nullptr;
+        SCIR_TRY(current_device_state(&st));
+        std::lock_guard<std::mutex> lk(st->mu);
+        return fir_mfma_ragged_launch(x, b, y, plan->mitems, plan->n_items, plan->counter,
+                                      plan->precision, plan->kmax, plan->lds, plan->grid, s);
+    }
+    if (mode == kDirectExact)
         hipLaunchKernelGGL(fir_ragged_kernel<true>, dim3(plan->grid), dim3(kThreads), plan->lds,
                            s, x, b, y, plan->items, plan->n_items, plan->counter);
     else
@@ -237,8 +299,8 @@ int scir_hip_ragged_plan_execute_per_signal(scir_hip_ragged_plan plan, const flo
     clear_error();
     if (!plan) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "plan is null");
     if (!x || !y || !b) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "null pointer argument");
-    bool exact = false;
-    SCIR_TRY(ragged_precision(plan->precision, &exact));
+    RaggedMode mode = kDirect;
+    SCIR_TRY(ragged_mode(plan->precision, &mode));
     DeviceState *st = nullptr;
     SCIR_TRY(current_device_state(&st));
     std::lock_guard<std::mutex> lk(st->mu);
@@ -247,8 +309,12 @@ int scir_hip_ragged_plan_execute_per_signal(scir_hip_ragged_plan plan, const flo
         const int64_t len = plan->offs[i + 1] - plan->offs[i];
         const int K = (int)(plan->boffs[i + 1] - plan->boffs[i]);
         if (len == 0) continue;
-        SCIR_TRY(fir_direct_dev(x + plan->offs[i], len, b + plan->boffs[i], K, 1, len,
-                                y + plan->offs[i], len, exact, false, s, st));
+        if (mode == kMfma)
+            SCIR_TRY(fir_mfma_dev(x + plan->offs[i], len, b + plan->boffs[i], K, 1, len,
+                                  y + plan->offs[i], len, plan->precision, false, s, st));
+        else
+            SCIR_TRY(fir_direct_dev(x + plan->offs[i], len, b + plan->boffs[i], K, 1, len,
+                                    y + plan->offs[i], len, mode == kDirectExact, false, s, st));
     }
     return SCIR_HIP_OK;
 }
@@ -258,6 +324,7 @@ int scir_hip_ragged_plan_destroy(scir_hip_ragged_plan plan)
     clear_error();
     if (!plan) return SCIR_HIP_OK;
     if (plan->items) (void)hipFree(plan->items);
+    if (plan->mitems) (void)hipFree(plan->mitems);
     if (plan->counter) (void)hipFree(plan->counter);
     delete plan;
     return SCIR_HIP_OK;

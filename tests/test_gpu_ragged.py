@@ -3,7 +3,10 @@
 Reference: per-signal FP64 oracle (oracle.lfilter_ragged_f64) within the
 (K_s+1)*2^-23*sum|b||x| bound; the f32_exact mode bit for bit against the
 reference loop per signal; the persistent launch against one dense launch
-per signal (same arithmetic: identical bits)."""
+per signal (same arithmetic: identical bits).  The MFMA precisions (bf16x3,
+fp16) run a persistent Toeplitz contraction whose per-output arithmetic is
+that of the dense MFMA kernel, so they too equal the per-signal dense launches
+bit for bit; bf16x3 is held to relative L2 error <= 1e-6 against FP64."""
 from __future__ import annotations
 
 import numpy as np
@@ -79,6 +82,47 @@ def test_persistent_equals_per_signal(hip):
                                       run(offs, boffs, x, b, prec, per_signal=True))
 
 
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16"])
+@pytest.mark.parametrize("seed,S,max_len,max_k", [(11, 1, 10, 3), (12, 30, 20000, 40),
+                                                  (13, 60, 40000, 1100), (14, 200, 3000, 300)])
+def test_mfma_ragged_equals_dense_per_signal(hip, seed, S, max_len, max_k, precision):
+    """Empty signals, K = 0, lengths 1 / 2047 / 2048 / 2049 and segments cut
+    mid-item: the persistent MFMA launch equals one dense MFMA launch per
+    signal, bit for bit."""
+    offs, boffs, x, b = make_case(seed, S, max_len, max_k)
+    y = run(offs, boffs, x, b, precision)
+    assert not np.isnan(y).any()
+    np.testing.assert_array_equal(y, run(offs, boffs, x, b, precision, per_signal=True))
+
+
+@pytest.mark.parametrize("precision,tol", [("bf16x3", 1e-6), ("fp16", 2e-3)])
+def test_mfma_ragged_rel_l2(hip, oracle, precision, tol):
+    offs, boffs, x, b = make_case(15, 40, 30000, 1023, allow_empty=False)
+    y = run(offs, boffs, x, b, precision)
+    y64 = oracle.lfilter_ragged_f64(x, offs, b, boffs)
+    assert oracle.rel_l2(y, y64) <= tol
+
+
+def test_mfma_ragged_unaligned_signals(hip, oracle):
+    """Signals starting at odd sample offsets take the scalar load/store path."""
+    offs = np.array([0, 3, 9004, 9005, 30001], dtype=np.int64)
+    boffs = np.array([0, 5, 600, 601, 1700], dtype=np.int64)
+    r = rng(16)
+    x = r.standard_normal(int(offs[-1])).astype(np.float32)
+    b = r.standard_normal(int(boffs[-1])).astype(np.float32)
+    y = run(offs, boffs, x, b, "bf16x3")
+    np.testing.assert_array_equal(y, run(offs, boffs, x, b, "bf16x3", per_signal=True))
+    assert oracle.rel_l2(y, oracle.lfilter_ragged_f64(x, offs, b, boffs)) <= 1e-6
+
+
+def test_fft_precision_rejected_for_ragged(hip):
+    from scir_amd import _lib
+    from scir_amd.ragged import RaggedPlan
+    with pytest.raises(_lib.HipError) as e:
+        RaggedPlan(np.array([0, 5]), np.array([0, 2]), "fft32")
+    assert e.value.status == _lib.ERR_UNSUPPORTED
+
+
 def test_invalid_offsets_rejected(hip):
     from scir_amd import _lib
     from scir_amd.ragged import RaggedPlan
@@ -88,14 +132,15 @@ def test_invalid_offsets_rejected(hip):
         RaggedPlan(np.array([1, 5]), np.array([0, 1]))
 
 
-def test_graph_replay_matches_eager(hip):
+@pytest.mark.parametrize("precision", ["f32", "bf16x3"])
+def test_graph_replay_matches_eager(hip, precision):
     """Capture reset + persistent launch; replay several times with eager
     work and synchronisation between replays; every replay must equal the
     eager result bit for bit."""
     import torch
     from scir_amd.ragged import Graph, RaggedPlan
     offs, boffs, x, b = make_case(8, 50, 10000, 500, allow_empty=False)
-    plan = RaggedPlan(offs, boffs, "f32")
+    plan = RaggedPlan(offs, boffs, precision)
     xt = torch.tensor(x, device="cuda")
     bt = torch.tensor(b, device="cuda")
     eager = plan.execute(xt, bt)
@@ -144,4 +189,17 @@ def test_full_size_ragged_workload(hip, oracle):
         k = taps[boffs[s]:boffs[s + 1]]
         y64 = oracle.lfilter_f64(xs, k)[0]
         assert np.all(np.abs(y[seg].cpu().numpy() - y64) <= oracle.fir_error_bound(xs, k)[0])
+    plan.close()
+    # The same workload on MFMA (BF16x3): persistent vs per-signal launches,
+    # and relative L2 error <= 1e-6 against FP64 on the same signals.
+    plan = RaggedPlan(offs, boffs, "bf16x3")
+    y = plan.execute(x, bt)
+    y2 = plan.execute_per_signal(x, bt)
+    torch.cuda.synchronize()
+    assert torch.equal(y, y2)
+    for s in pick:
+        seg = slice(int(offs[s]), int(offs[s + 1]))
+        xs = x[seg].cpu().numpy()
+        y64 = oracle.lfilter_f64(xs, taps[boffs[s]:boffs[s + 1]])[0]
+        assert oracle.rel_l2(y[seg].cpu().numpy(), y64) <= 1e-6, s
     plan.close()
