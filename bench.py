@@ -313,7 +313,10 @@ def ragged_roofline(prec, macs, issued, ms):
                     unit="TFLOP/s", frac=round(ach / BF16_PEAK_TFLOPS, 4),
                     traffic=load_traffic(f"fir_ragged_{prec}_ragged_pmc"),
                     mfma_issued_tflops=round(iss, 1),
-                    mfma_issued_frac=round(iss / BF16_PEAK_TFLOPS, 4))
+                    mfma_issued_frac=round(iss / BF16_PEAK_TFLOPS, 4),
+                    mfma_busy_frac=load_profile(f"fir_ragged_{prec}_ragged_util",
+                                                "mfma_busy_frac"),
+                    sclk_ghz=load_profile(f"fir_ragged_{prec}_ragged_util", "sclk_ghz"))
     return dict(bound="fp32", achieved=round(ach, 2), peak=FP32_PEAK_TFLOPS,
                 unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 4),
                 traffic=load_traffic(f"fir_ragged_{prec}_ragged_pmc"))

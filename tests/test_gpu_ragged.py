@@ -123,6 +123,16 @@ def test_fft_precision_rejected_for_ragged(hip):
     assert e.value.status == _lib.ERR_UNSUPPORTED
 
 
+def test_mfma_ragged_tap_limit(hip):
+    """BF16x3 ragged plans take taps up to K = 5601 (the LDS of one CU)."""
+    from scir_amd import _lib
+    from scir_amd.ragged import RaggedPlan
+    RaggedPlan(np.array([0, 10]), np.array([0, 5601]), "bf16x3").close()
+    with pytest.raises(_lib.HipError) as e:
+        RaggedPlan(np.array([0, 10]), np.array([0, 5602]), "bf16x3")
+    assert e.value.status == _lib.ERR_UNSUPPORTED
+
+
 def test_invalid_offsets_rejected(hip):
     from scir_amd import _lib
     from scir_amd.ragged import RaggedPlan
