@@ -56,7 +56,9 @@ __device__ __forceinline__ float mac<false>(float acc, float t, float xv)
 // acc[r] += sum_{kk < C} bk[kk] * x[i_r - k0 - kk], where the LDS window
 // starts OFF floats below the 4-aligned index `wa`:
 //   x[i_r - k0 - kk] == xs[wa + OFF + r + C - kk].
-template <int R, int C, int OFF, bool EXACT>
+// With FIRST the chunk starts the sums: its first tap is accumulated onto 0
+// (the same arithmetic as zeroed accumulators, with no zeroing instructions).
+template <int R, int C, int OFF, bool EXACT, bool FIRST = false>
 __device__ __forceinline__ void taps_chunk(float (&acc)[R], const float *xs, int wa,
                                            const float *__restrict__ bk)
 {
@@ -74,7 +76,8 @@ __device__ __forceinline__ void taps_chunk(float (&acc)[R], const float *xs, int
     for (int kk = 0; kk < C; ++kk) {
         const float t = bk[kk];
 #pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = mac<EXACT>(acc[r], t, w[OFF + r + C - kk]);
+        for (int r = 0; r < R; ++r)
+            acc[r] = mac<EXACT>(FIRST && kk == 0 ? 0.0f : acc[r], t, w[OFF + r + C - kk]);
     }
 }
 
@@ -115,12 +118,24 @@ __device__ __forceinline__ void stage_row(float *xs, const float *__restrict__ x
 
 // acc[r] += sum_{k' < kcount} bs[k'] * x[i_r - k'] with the window in LDS:
 // x[i_r - k'] == xs[H + lane_base + r - k'].  Chunks run in ascending k'.
-template <int R, bool EXACT>
+// With INIT the sums start from zero instead (acc[r] = sum ...).
+template <int R, bool EXACT, bool INIT = false>
 __device__ __forceinline__ void taps_from_lds(float (&acc)[R], const float *xs, int H,
                                               int lane_base, int kcount,
                                               const float *__restrict__ bs)
 {
+    // INIT: acc holds no partial sums on entry; the first 32-tap chunk (or,
+    // for fewer taps, explicit zeros) starts them.
     int kc = 0;
+    if (INIT) {
+        if (kcount >= kChunk) {
+            taps_chunk<R, kChunk, 0, EXACT, true>(acc, xs, H + lane_base - kChunk, bs);
+            kc = kChunk;
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = 0.0f;
+        }
+    }
     for (; kc + kChunk <= kcount; kc += kChunk)
         taps_chunk<R, kChunk, 0, EXACT>(acc, xs, H + lane_base - kc - kChunk, bs + kc);
     const int rem = kcount - kc;
@@ -136,15 +151,15 @@ __device__ __forceinline__ void taps_from_lds(float (&acc)[R], const float *xs, 
         taps_chunk<R, 4, 0, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
         kc += 4;
     }
-    if (rem & 2) {  // kc % 4 == 0 here; window starts 2 below a 4-aligned index
+    // kc % 4 == 0 here.  The last one or two taps read windows that start 1,
+    // 2 or 3 below a 4-aligned index; each case has its own branch, so no
+    // window registers are merged (copied) between the branches.
+    if (rem & 2) {  // window starts 2 below the aligned index
         taps_chunk<R, 2, 2, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
-        kc += 2;
-    }
-    if (rem & 1) {
-        if (kc & 2)  // kc % 4 == 2: window starts 1 below the aligned index
-            taps_chunk<R, 1, 1, EXACT>(acc, xs, H + lane_base - kc - 2, bs + kc);
-        else         // kc % 4 == 0: window starts 3 below the aligned index
-            taps_chunk<R, 1, 3, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
+        if (rem & 1)  // kc + 2: window starts 1 below the aligned index
+            taps_chunk<R, 1, 1, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc + 2);
+    } else if (rem & 1) {  // window starts 3 below the aligned index
+        taps_chunk<R, 1, 3, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
     }
 }
 

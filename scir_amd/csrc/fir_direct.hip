@@ -114,9 +114,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_pipe_kernel(
         __syncthreads();
         if (it + 1 < ntiles) fetch(tfirst + it + 1);
         float acc[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = 0.0f;
-        taps_from_lds<R, EXACT>(acc, buf, H, lane_base, K, b);
+        taps_from_lds<R, EXACT, true>(acc, buf, H, lane_base, K, b);
         store_outputs<R>(acc, yrow, (tfirst + it) * T + lane_base, N, vec_y != 0, rev != 0);
     }
 }
@@ -220,9 +218,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_dma_kernel(
         if (it + NB - 1 < ntiles) issue(tfirst + it + NB - 1, (it + NB - 1) % NB);
         const float *buf = lds + (it % NB) * buf_chunks * 4;
         float acc[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = 0.0f;
-        taps_from_lds<R, EXACT>(acc, buf, H, lane_base, K, b);
+        taps_from_lds<R, EXACT, true>(acc, buf, H, lane_base, K, b);
         const int64_t i0 = (tfirst + it) * T + lane_base;
 #pragma unroll
         for (int r = 0; r < R; r += 4)
