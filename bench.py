@@ -260,7 +260,7 @@ def run_dense(args):
         for other in ("f32", "fp16", "bf16x3", "fft32"):
             if other == prec or (other == "fft32" and K > 8192):
                 continue
-            # 3 untimed calls: the first FFT32 call also builds the spectrum and
+            # 3 untimed calls: the first FFT32 call on a device also builds its
             # twiddle tables, and the clock settles after the switch of kernel.
             ms_o, _ = timed(lambda: device.fir(x, b, other, out=y), max(5, args.steps // 2),
                             3, stream)
