@@ -134,8 +134,24 @@ def cpu_baseline(rows_np, b, budget_s):
     dt = time.perf_counter() - t0
     n = rows_np.shape[1]
     return dict(value=round(done * n / dt / 1e9, 6), unit="Gsamples/s", cores=1, kind="port",
+                host=host_cpu(),
                 sample=f"{done} whole rows x {n} samples of the same workload, "
                        f"1 thread, {dt:.1f} s")
+
+
+def host_cpu():
+    """The host CPU model (/proc/cpuinfo) and the logical CPU counts."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return dict(model=model, nproc=os.cpu_count(), usable=usable)
 
 
 def cpu_all_cores(rows_np, b, budget_s):
@@ -162,7 +178,7 @@ def cpu_all_cores(rows_np, b, budget_s):
             break
     dt = time.perf_counter() - t0
     n = rows_np.size * reps
-    return dict(value=round(n / dt / 1e9, 6), unit="Gsamples/s", cores=threads,
+    return dict(value=round(n / dt / 1e9, 6), unit="Gsamples/s", cores=threads, host=host_cpu(),
                 impl="library CPU arm (reference loop order, row-partitioned threads)",
                 sample=f"{reps} passes over {rows_np.shape[0]} whole rows x {rows_np.shape[1]} "
                        f"samples, {threads} threads, {dt:.1f} s")
