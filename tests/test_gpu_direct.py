@@ -90,6 +90,22 @@ def test_f32_within_bound_of_fp64(hip, oracle, variant, K):
         assert np.all(excess <= 0), f"K={K} N={N}: max excess {excess.max()}"
 
 
+def test_cpu_config_workload(hip, oracle):
+    """The reference's CPU case through the drop-in host tier: x [64, 16384]
+    ~ N(0,1) f32 from a seed, the 33-tap Hamming low-pass (cutoff 0.2 of
+    Nyquist, unit DC gain).  f32_exact equals the reference loop bit for bit;
+    f32 and AUTO (whichever kernel a loaded tuning table makes it take) lie
+    within (K+1)*2^-23*sum|b||x| of FP64."""
+    from scir_amd import design
+    x = rng(0x5C120000).standard_normal((64, 16384)).astype(np.float32)
+    b = design.lowpass(33, 0.2).astype(np.float32)
+    np.testing.assert_array_equal(host_fir(hip, x, b, "f32_exact"),
+                                  oracle.fir_ref_f32(x, b[::-1]))
+    y64, bound = oracle.lfilter_f64(x, b), oracle.fir_error_bound(x, b)
+    for prec in ("f32", "auto"):
+        assert np.all(np.abs(host_fir(hip, x, b, prec) - y64) <= bound), prec
+
+
 def test_k0_zero_and_empty_shapes(hip):
     x = rng(1).standard_normal((3, 50)).astype(np.float32)
     y = host_fir(hip, x, np.zeros(0, np.float32), "f32")

@@ -45,6 +45,18 @@ def test_cpu_arm_bitwise_vs_oracle(oracle, B, N, K):
         np.testing.assert_array_equal(y, oracle.fir_ref_f32(x, taps))
 
 
+def test_cpu_config_workload_cpu_arm(oracle):
+    """The reference's CPU case (x [64, 16384] ~ N(0,1) f32, 33-tap Hamming
+    low-pass, cutoff 0.2 of Nyquist) on the library's CPU arm: bit for bit
+    the reference loop, and within (K+1)*2^-23*sum|b||x| of FP64."""
+    from scir_amd import design
+    x = rng(0x5C120000).standard_normal((64, 16384)).astype(np.float32)
+    b = design.lowpass(33, 0.2).astype(np.float32)
+    y = scir_amd.fir1d_batched_f32(x, b[::-1].copy())
+    np.testing.assert_array_equal(y, oracle.fir_ref_f32(x, b[::-1]))
+    assert np.all(np.abs(y - oracle.lfilter_f64(x, b)) <= oracle.fir_error_bound(x, b))
+
+
 def test_cpu_arm_accepts_non_contiguous_input(oracle):
     r = rng(9)
     base = r.standard_normal((6, 50)).astype(np.float32)
