@@ -204,6 +204,37 @@ def host_tier(x, b_np, reps=3):
                      f"{reps} calls; not the headline")
 
 
+def reference_cpu_case(stream):
+    """The reference's own CPU-runnable case, [64, 16384] f32 ~ N(0,1) with the
+    33-tap Hamming low-pass: the oracle's 1-thread restatement of the reference
+    loop over the whole case, the library's CPU arm on all usable threads, the
+    device tier (inputs resident) and the drop-in host tier.  A parity case
+    reported beside the headline, not a bench line of its own."""
+    import torch
+    from scir_amd import design, device
+    from oracle import oracle as orc
+    x_np = np.random.default_rng(0x5C120000).standard_normal((64, 16384)).astype(np.float32)
+    b_np = design.lowpass(33, 0.2).astype(np.float32)
+    n = x_np.size
+    reps, t0 = 0, time.perf_counter()
+    while reps < 3 or time.perf_counter() - t0 < 1.0:
+        y_ref = orc.fir_ref_f32(x_np, b_np[::-1])
+        reps += 1
+    cpu1 = n * reps / (time.perf_counter() - t0) / 1e9
+    x = torch.tensor(x_np, device="cuda")
+    b = torch.tensor(b_np, device="cuda")
+    y = torch.empty_like(x)
+    ms, _ = timed(lambda: device.fir(x, b, "f32_exact", out=y), 200, 50, stream)
+    exact = bool(np.array_equal(y.cpu().numpy(), y_ref))
+    return dict(workload="[64, 16384] f32, 33-tap Hamming low-pass 0.2 of Nyquist",
+                cpu_1thread_gsamples_s=round(cpu1, 4),
+                cpu_all_cores=cpu_all_cores(x_np, b_np, 1.0),
+                gpu_f32_exact_gsamples_s=round(n / (ms * 1e-3) / 1e9, 2),
+                gpu_us_per_call=round(ms * 1e3, 1),
+                gpu_bitwise_equal_reference_loop=exact,
+                host_tier=host_tier(x, b_np, reps=5))
+
+
 def dense_setup(name):
     import torch
     from scir_amd import device
@@ -303,6 +334,8 @@ def run_dense(args):
         out["cpu_baseline"] = cpu_baseline(x[:nrows].cpu().numpy(), b_np, args.cpu_budget)
         out["cpu_all_cores"] = cpu_all_cores(x[:min(B, 64)].cpu().numpy(), b_np,
                                              min(5.0, args.cpu_budget))
+        if not args.quick:
+            out["reference_cpu_case"] = reference_cpu_case(stream)
     return out
 
 

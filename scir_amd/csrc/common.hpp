@@ -84,6 +84,9 @@ int check_dense_args(const void *x, int64_t ldx, const void *b, int32_t K, int64
 
 // Launch-status check after a kernel launch.
 int check_launch(const char *what);
+// Lets kernel `fn` take `bytes` of dynamic LDS on the current device (set
+// once per device and kernel; thread-safe).
+int allow_dynamic_lds(const void *fn, int bytes);
 
 // ---- dense entry points shared across translation units -----------------
 // `rev` runs the filter back to front: y[N-1-i] = sum_k b[k] x[N-1-i+k], the

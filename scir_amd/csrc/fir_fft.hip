@@ -348,17 +348,12 @@ int launch_fft(const float *x, int64_t ldx, const float *b, int K, int64_t B, in
     SCIR_TRY(ensure_buffer(sc.fft_spectrum, (size_t)NF * sizeof(float2), s));
     float2 *H = static_cast<float2 *>(sc.fft_spectrum.ptr);
     constexpr size_t lds = lds_bytes<LOGN>();
-    static bool attr_set = false;  // guarded by st->mu
-    if (!attr_set) {
-        SCIR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&fir_fft_kernel<LOGN, false>),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        SCIR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&fir_fft_kernel<LOGN, true>),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        SCIR_HIP_TRY(hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&fir_fft_spectrum_kernel<LOGN>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        attr_set = true;
-    }
+    SCIR_TRY(allow_dynamic_lds(reinterpret_cast<const void *>(&fir_fft_kernel<LOGN, false>),
+                               (int)lds));
+    SCIR_TRY(allow_dynamic_lds(reinterpret_cast<const void *>(&fir_fft_kernel<LOGN, true>),
+                               (int)lds));
+    SCIR_TRY(allow_dynamic_lds(reinterpret_cast<const void *>(&fir_fft_spectrum_kernel<LOGN>),
+                               (int)lds));
     hipLaunchKernelGGL((fir_fft_spectrum_kernel<LOGN>), dim3(1), dim3(T), lds, s, b, K, tq, H);
     SCIR_TRY(check_launch("fir_fft_spectrum_kernel"));
     const int64_t V = NF - K + 1;

@@ -583,12 +583,8 @@ int launch_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, i
     const int tab_offset = S * split_stride;
     const int copy_bytes = tab_copy_bytes(m_hi, TW);
     const size_t lds = mfma_lds_bytes<S, TW>(K);
-    static bool attr_set = false;  // guarded by the device mutex held by the caller
-    if (!attr_set) {
-        SCIR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&fir_mfma_kernel<S, TW>),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds));
-        attr_set = true;
-    }
+    SCIR_TRY(
+        allow_dynamic_lds(reinterpret_cast<const void *>(&fir_mfma_kernel<S, TW>), kMaxLds));
     const int64_t seg_len = 32 * (int64_t)SEG_BLOCKS;
     const int64_t segs = (N + seg_len - 1) / seg_len;
     if (segs > 0x7fffffff)
@@ -679,13 +675,8 @@ int ragged_launch(const float *x, const float *b, float *y, const MfmaRaggedItem
     const RaggedLayout<S> lay(kmax);
     if (lay.bytes != lds || lds > (size_t)kMaxLds)
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "ragged MFMA layout mismatch");
-    static bool attr_set = false;  // guarded by the device mutex held by the caller
-    if (!attr_set) {
-        SCIR_HIP_TRY(hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&fir_mfma_ragged_kernel<S, kRaggedTW>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds));
-        attr_set = true;
-    }
+    SCIR_TRY(allow_dynamic_lds(
+        reinterpret_cast<const void *>(&fir_mfma_ragged_kernel<S, kRaggedTW>), kMaxLds));
     hipLaunchKernelGGL((fir_mfma_ragged_kernel<S, kRaggedTW>), dim3((unsigned)grid),
                        dim3(kThreads), lds, stream, x, b, y, items, n_items, counter,
                        lay.split_stride, lay.tab_offset, lay.copy_bytes, lay.slot_offset);
@@ -712,7 +703,7 @@ int fir_mfma_ragged_occupancy(int precision, size_t lds, int *per_cu)
     const void *fn = precision == SCIR_HIP_PREC_BF16X3
                          ? reinterpret_cast<const void *>(&fir_mfma_ragged_kernel<3, kRaggedTW>)
                          : reinterpret_cast<const void *>(&fir_mfma_ragged_kernel<1, kRaggedTW>);
-    SCIR_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds));
+    SCIR_TRY(allow_dynamic_lds(fn, kMaxLds));
     SCIR_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kThreads, lds));
     *per_cu = std::max(1, n);
     return SCIR_HIP_OK;

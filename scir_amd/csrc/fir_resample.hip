@@ -142,14 +142,10 @@ int resample_launch(const double *x, int64_t ldx, int64_t B, int64_t N, int32_t 
     if (lds > 160 * 1024)
         return fail(SCIR_HIP_ERR_UNSUPPORTED, "K=%d, up=%d, down=%d need too much LDS", K, up,
                     down);
-    static bool attr_set = false;  // guarded by the device mutex
-    if (!attr_set) {
-        SCIR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&resample_poly_kernel<true>),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        SCIR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&resample_poly_kernel<false>),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_set = true;
-    }
+    SCIR_TRY(allow_dynamic_lds(reinterpret_cast<const void *>(&resample_poly_kernel<true>),
+                               160 * 1024));
+    SCIR_TRY(allow_dynamic_lds(reinterpret_cast<const void *>(&resample_poly_kernel<false>),
+                               160 * 1024));
     // Every output of a lane has one phase when up divides kThreads * down.
     const bool uni = ((int64_t)kThreads * down) % up == 0;
     const int64_t tiles = (M + kT - 1) / kT;

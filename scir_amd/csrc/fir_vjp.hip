@@ -201,12 +201,8 @@ int vjp_db_run(const float *g, int64_t ldg, const float *x, int64_t ldx, int32_t
     const size_t lds = lds_floats * sizeof(float);
     if (lds > 160 * 1024)
         return fail(SCIR_HIP_ERR_UNSUPPORTED, "K=%d is too long for the dL/db kernel", K);
-    static bool attr_set = false;  // guarded by st->mu
-    if (!attr_set) {
-        SCIR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&fir_vjp_db_partial_kernel),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_set = true;
-    }
+    SCIR_TRY(allow_dynamic_lds(reinterpret_cast<const void *>(&fir_vjp_db_partial_kernel),
+                               160 * 1024));
     StreamScratch &sc = scratch_for(st, s);
     SCIR_TRY(ensure_buffer(sc.partials, (size_t)nwg * K * sizeof(float), s));
     float *partial = static_cast<float *>(sc.partials.ptr);

@@ -9,7 +9,10 @@
 #include "common.hpp"
 
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <utility>
 #include <vector>
 
 namespace scir {
@@ -114,6 +117,23 @@ int check_launch(const char *what)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return fail(SCIR_HIP_ERR_BACKEND, "launch of %s failed: %s", what, hipGetErrorString(e));
+    return SCIR_HIP_OK;
+}
+
+int allow_dynamic_lds(const void *fn, int bytes)
+{
+    // The attribute belongs to the kernel on one device: remember the largest
+    // value set per (device, kernel).
+    static std::mutex mu;
+    static std::map<std::pair<int, const void *>, int> done;
+    int dev = -1;
+    SCIR_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    int &have = done[{dev, fn}];
+    if (have < bytes) {
+        SCIR_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        have = bytes;
+    }
     return SCIR_HIP_OK;
 }
 
