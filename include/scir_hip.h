@@ -74,10 +74,12 @@ typedef enum scir_hip_precision {
     /* Fastest of F32 / BF16X3 for the shape (autotuned table when loaded,
      * else built-in cut-over); never FP16 or FFT32. */
     SCIR_HIP_PREC_AUTO = 4,
-    /* FP32 FFT overlap-save (K <= 8192): cost O(log K) per output instead of
-     * O(K); the error is normwise (about log2(NF) * 2^-24 * |x|_2 |b|_1 over
-     * each NF = 4096, 8192 or 16384 sample segment), not elementwise.  x and
-     * y must not be the same array. */
+    /* FP32 FFT overlap-save: cost O(log K) per output instead of O(K) for
+     * K <= 8192; longer filters run as blocks of 8192 taps (partitioned
+     * overlap-save, one pass over the rows per block, outputs summed).  The
+     * error is normwise (about log2(NF) * 2^-24 * |x|_2 |b|_1 over each
+     * NF = 4096, 8192 or 16384 sample segment), not elementwise.  x and y
+     * must not be the same array. */
     SCIR_HIP_PREC_FFT32 = 5
 } scir_hip_precision;
 
@@ -301,7 +303,7 @@ SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
  *                     next segment's samples load while the current one computes
  *   "auto.mfma_min_k" smallest K for which AUTO picks BF16x3 MFMA
  *   "auto.fft"        1: AUTO may pick the FFT32 path (normwise error) from
- *                     auto.fft_min_k up (K <= 8192, x != y); 0 (default): never
+ *                     auto.fft_min_k up (x != y); 0 (default): never
  *   "auto.fft_min_k"  smallest K for which AUTO (with auto.fft) picks FFT32;
  *                     the autotuner measures it (the K from which FFT32 beats
  *                     BF16x3 MFMA on long rows)

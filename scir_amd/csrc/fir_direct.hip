@@ -371,7 +371,7 @@ int fir_dense_dispatch(const float *x, int64_t ldx, const float *b, int32_t K, i
         return fir_fft_dev(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
     case SCIR_HIP_PREC_AUTO: {
         const TuneChoice tc = tune_lookup(K, N);
-        if (tc.auto_fft && K >= tc.auto_fft_min_k && K <= 8192 && x != y)
+        if (tc.auto_fft && K >= tc.auto_fft_min_k && x != y)
             return fir_fft_dev(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
         if (K >= tc.auto_mfma_min_k)
             return fir_mfma_dev(x, ldx, b, K, B, N, y, ldy, SCIR_HIP_PREC_BF16X3, rev, stream, st);

@@ -239,12 +239,14 @@ __global__ __launch_bounds__(Plan<LOGN>::T) void fir_fft_spectrum_kernel(
 
 // Segment s of the (row, segment) list: row s / spr, outputs [o, o + V),
 // V = NF - K + 1, o = (s % spr) * V.  REV reads and writes every row back to
-// front (the anti-causal correlation of the dL/dx adjoint).
+// front (the anti-causal correlation of the dL/dx adjoint).  With `accumulate`
+// the outputs are added to y instead of stored (the later tap blocks of a
+// partitioned filter).
 template <int LOGN, bool REV>
 __global__ __launch_bounds__(Plan<LOGN>::T) void fir_fft_kernel(
     const float *__restrict__ x, int64_t ldx, float *__restrict__ y, int64_t ldy, int64_t N,
     int K, const float2 *__restrict__ H, const float2 *__restrict__ tq_g, int64_t spr,
-    int64_t nseg)
+    int64_t nseg, int accumulate)
 {
     constexpr int NF = 1 << LOGN, T = Plan<LOGN>::T;
     extern __shared__ float2 lds_f2[];
@@ -282,18 +284,33 @@ __global__ __launch_bounds__(Plan<LOGN>::T) void fir_fft_kernel(
     if (has1 && oa + V <= N && ob + V <= N) {
         float *qa = REV ? ya + (N - 1 - oa) : ya + oa;
         float *qb = REV ? yb + (N - 1 - ob) : yb + ob;
+        if (accumulate) {
 #pragma unroll 4
-        for (int v = tid; v < V; v += T) {
-            const float2 o = d[pad(K - 1 + v)];
-            qa[REV ? -v : v] = o.x;
-            qb[REV ? -v : v] = o.y;
+            for (int v = tid; v < V; v += T) {
+                const float2 o = d[pad(K - 1 + v)];
+                qa[REV ? -v : v] += o.x;
+                qb[REV ? -v : v] += o.y;
+            }
+        } else {
+#pragma unroll 4
+            for (int v = tid; v < V; v += T) {
+                const float2 o = d[pad(K - 1 + v)];
+                qa[REV ? -v : v] = o.x;
+                qb[REV ? -v : v] = o.y;
+            }
         }
     } else {
         for (int v = tid; v < V; v += T) {
             const float2 o = d[pad(K - 1 + v)];
             const int64_t ta = oa + v, tb = ob + v;
-            if (ta < N) ya[REV ? N - 1 - ta : ta] = o.x;
-            if (has1 && tb < N) yb[REV ? N - 1 - tb : tb] = o.y;
+            if (ta < N) {
+                float &q = ya[REV ? N - 1 - ta : ta];
+                q = accumulate ? q + o.x : o.x;
+            }
+            if (has1 && tb < N) {
+                float &q = yb[REV ? N - 1 - tb : tb];
+                q = accumulate ? q + o.y : o.y;
+            }
         }
     }
 }
@@ -339,7 +356,7 @@ int twiddle_table(DeviceState *st, int logn, hipStream_t s, const float2 **out)
 
 template <int LOGN>
 int launch_fft(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N, float *y,
-               int64_t ldy, bool rev, hipStream_t s, DeviceState *st)
+               int64_t ldy, bool rev, bool accumulate, hipStream_t s, DeviceState *st)
 {
     constexpr int NF = 1 << LOGN, T = Plan<LOGN>::T;
     const float2 *tq = nullptr;
@@ -361,20 +378,22 @@ int launch_fft(const float *x, int64_t ldx, const float *b, int K, int64_t B, in
     const int64_t nseg = B * spr;
     const int64_t blocks = (nseg + 1) / 2;
     if (blocks > 0x7fffffff) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "batch too large");
+    const int acc = accumulate ? 1 : 0;
     if (rev)
         hipLaunchKernelGGL((fir_fft_kernel<LOGN, true>), dim3((unsigned)blocks), dim3(T), lds, s, x,
-                           ldx, y, ldy, N, K, H, tq, spr, nseg);
+                           ldx, y, ldy, N, K, H, tq, spr, nseg, acc);
     else
         hipLaunchKernelGGL((fir_fft_kernel<LOGN, false>), dim3((unsigned)blocks), dim3(T), lds, s,
-                           x, ldx, y, ldy, N, K, H, tq, spr, nseg);
+                           x, ldx, y, ldy, N, K, H, tq, spr, nseg, acc);
     return check_launch("fir_fft_kernel");
 }
 
-}  // namespace
-
-// Largest K the FFT path takes: the largest transform (2^14) keeps at least
-// half of each segment as outputs.
-constexpr int kFftMaxTaps = 8192;
+// Taps per block of the FFT path: the largest transform (2^14) keeps at
+// least half of each segment as outputs.  Longer filters are cut into blocks
+// of kFftBlockTaps taps (uniformly partitioned overlap-save): block j, taps
+// b[jP, jP + P), filters the input delayed by jP samples, and its outputs are
+// added to those of the blocks before it.
+constexpr int kFftBlockTaps = 8192;
 
 // FFT size for K taps: the smallest of 2^12, 2^13, 2^14 with NF >= 4K
 // (outputs per segment >= 3/4 NF), else 2^14.
@@ -385,21 +404,36 @@ int fft_log_size(int K)
     return 14;
 }
 
+int fft_block(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N, float *y,
+              int64_t ldy, bool rev, bool accumulate, hipStream_t stream, DeviceState *st)
+{
+    switch (fft_log_size(K)) {
+    case 12: return launch_fft<12>(x, ldx, b, K, B, N, y, ldy, rev, accumulate, stream, st);
+    case 13: return launch_fft<13>(x, ldx, b, K, B, N, y, ldy, rev, accumulate, stream, st);
+    default: return launch_fft<14>(x, ldx, b, K, B, N, y, ldy, rev, accumulate, stream, st);
+    }
+}
+
+}  // namespace
+
 int fir_fft_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N, float *y,
                 int64_t ldy, bool rev, hipStream_t stream, DeviceState *st)
 {
     if (K == 0)  // no taps: zeros, as every path gives
         return fir_direct_dev(x, ldx, b, K, B, N, y, ldy, false, rev, stream, st);
-    if (K < 0 || K > kFftMaxTaps)
-        return fail(SCIR_HIP_ERR_UNSUPPORTED, "the FFT path takes 1 <= K <= %d (K=%d)", kFftMaxTaps,
-                    K);
+    if (K < 0) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "negative tap count");
     if (x == y)
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "the FFT path does not filter in place");
-    switch (fft_log_size(K)) {
-    case 12: return launch_fft<12>(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
-    case 13: return launch_fft<13>(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
-    default: return launch_fft<14>(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
+    if (B == 0 || N == 0) return SCIR_HIP_OK;
+    for (int64_t j0 = 0; j0 < K && j0 < N; j0 += kFftBlockTaps) {
+        const int Kj = (int)std::min<int64_t>(kFftBlockTaps, K - j0);
+        // Forward: y[j0 + i] += sum_k b[j0 + k] x[i - k] for i < N - j0.  Back
+        // to front the same sums read x from sample j0 on and write y from 0.
+        const float *xs = rev ? x + j0 : x;
+        float *ys = rev ? y : y + j0;
+        SCIR_TRY(fft_block(xs, ldx, b + j0, Kj, B, N - j0, ys, ldy, rev, j0 > 0, stream, st));
     }
+    return SCIR_HIP_OK;
 }
 
 }  // namespace scir

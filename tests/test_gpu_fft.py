@@ -5,7 +5,9 @@ about log2(NF) * 2^-24 * |x_seg|_2 * |b|_1.  The test bound per output is
 c * log2(NF) * 2^-24 * |b|_1 * |x|_2 over the row (a superset of every
 segment, so the bound is looser than the per-segment one) with c = 4, and the
 relative L2 error of each case with rows of at least 4K samples must stay
-under 1e-6 (the accuracy bar the long-filter path is held to)."""
+under 1e-6 (the accuracy bar the long-filter path is held to).  Filters
+longer than 8192 taps run as blocks of 8192 taps (partitioned overlap-save),
+whose outputs are summed; the same bound holds."""
 import numpy as np
 import pytest
 import torch
@@ -90,10 +92,38 @@ def test_fft_adjoint_dx():
     assert orc.rel_l2(dx, dx64) <= 1e-6
 
 
+@pytest.mark.parametrize("K", [8193, 12000, 16384, 20001])
+@pytest.mark.parametrize("shape", [(2, 5000), (3, 40000), (1, 100003)])
+def test_fft_partitioned_long_filters(K, shape):
+    """K > 8192: blocks of 8192 taps, each on the input delayed by its first
+    tap, summed into y; rows shorter than a block offset skip that block."""
+    print(f"fft partitioned K={K} shape={shape}", flush=True)
+    rng = np.random.default_rng(K + 7 * shape[1])
+    x = rng.standard_normal(shape).astype(np.float32)
+    b = (rng.standard_normal(K) / np.sqrt(K)).astype(np.float32)
+    y = device.fir(torch.tensor(x, device="cuda:0"), torch.tensor(b, device="cuda:0"), "fft32")
+    torch.cuda.synchronize()
+    _check(y.cpu().numpy(), x, b)
+
+
+def test_fft_partitioned_adjoint_dx():
+    # the back-to-front run of a partitioned filter (dL/dx with K > 8192)
+    rng = np.random.default_rng(5)
+    B, N, K = 2, 60000, 10000
+    g = rng.standard_normal((B, N)).astype(np.float32)
+    x = rng.standard_normal((B, N)).astype(np.float32)
+    b = (rng.standard_normal(K) / 100).astype(np.float32)
+    dx = device.vjp_dx(torch.tensor(g, device="cuda:0"), torch.tensor(b, device="cuda:0"),
+                       "fft32").cpu().numpy()
+    dx64, _ = orc.vjp_f64(g, x, b)
+    bound = 4 * np.log2(16384) * EPS32 * np.sum(np.abs(b)) * np.linalg.norm(g, axis=1,
+                                                                           keepdims=True)
+    assert np.all(np.abs(dx - dx64) <= bound)
+    assert orc.rel_l2(dx, dx64) <= 1e-6
+
+
 def test_fft_refusals():
     x = torch.zeros((1, 100), device="cuda:0")
-    with pytest.raises(Exception):
-        device.fir(x, torch.zeros(8193, device="cuda:0"), "fft32")
     with pytest.raises(Exception):
         device.fir(x, torch.ones(5, device="cuda:0"), "fft32", out=x)
     # K = 0 gives zeros like every path
