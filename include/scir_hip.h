@@ -66,13 +66,15 @@ typedef enum scir_hip_precision {
      * +0.0f): bit-identical to crates/scir-gpu/src/lib.rs:1134-1152. */
     SCIR_HIP_PREC_F32_EXACT = 1,
     /* Toeplitz-block contraction on MFMA: x and b split into three bf16 terms
-     * (hi+mid+lo), six cross products kept, FP32 accumulation. */
+     * (hi+mid+lo), six cross products kept, FP32 accumulation.  K <= 6689
+     * (FP16: K <= 23457); longer filters give SCIR_HIP_ERR_UNSUPPORTED. */
     SCIR_HIP_PREC_BF16X3 = 2,
     /* Toeplitz-block contraction on MFMA with one fp16 product (x and b
      * rounded to fp16), FP32 accumulation.  Reported, not accuracy-bound. */
     SCIR_HIP_PREC_FP16 = 3,
     /* Fastest of F32 / BF16X3 for the shape (autotuned table when loaded,
-     * else built-in cut-over); never FP16 or FFT32. */
+     * else built-in cut-over; F32 for K > 6689, beyond the MFMA kernel's
+     * LDS); never FP16, and FFT32 only when the "auto.fft" key is set. */
     SCIR_HIP_PREC_AUTO = 4,
     /* FP32 FFT overlap-save: cost O(log K) per output instead of O(K) for
      * K <= 8192; longer filters run as blocks of 8192 taps (partitioned

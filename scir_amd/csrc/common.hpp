@@ -99,6 +99,8 @@ int fir_fft_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, i
 int fir_mfma_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
                  float *y, int64_t ldy, int precision, bool rev, hipStream_t stream,
                  DeviceState *st);
+// Whether the MFMA kernel's LDS holds K taps (one tile per wave).
+bool fir_mfma_takes(int precision, int K);
 int fir_dense_dispatch(const float *x, int64_t ldx, const float *b, int32_t K, int64_t B,
                        int64_t N, float *y, int64_t ldy, int32_t precision, bool rev,
                        hipStream_t stream, DeviceState *st);

@@ -373,7 +373,8 @@ int fir_dense_dispatch(const float *x, int64_t ldx, const float *b, int32_t K, i
         const TuneChoice tc = tune_lookup(K, N);
         if (tc.auto_fft && K >= tc.auto_fft_min_k && x != y)
             return fir_fft_dev(x, ldx, b, K, B, N, y, ldy, rev, stream, st);
-        if (K >= tc.auto_mfma_min_k)
+        // BF16x3 MFMA from the cut-over up, while its LDS holds the taps.
+        if (K >= tc.auto_mfma_min_k && fir_mfma_takes(SCIR_HIP_PREC_BF16X3, K))
             return fir_mfma_dev(x, ldx, b, K, B, N, y, ldy, SCIR_HIP_PREC_BF16X3, rev, stream, st);
         return fir_direct_dev(x, ldx, b, K, B, N, y, ldy, false, rev, stream, st);
     }

@@ -635,6 +635,14 @@ int run_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int6
 
 }  // namespace
 
+bool fir_mfma_takes(int precision, int K)
+{
+    if (K <= 0) return true;
+    const size_t need = precision == SCIR_HIP_PREC_BF16X3 ? mfma_lds_bytes<3, 1>(K)
+                                                          : mfma_lds_bytes<1, 1>(K);
+    return need <= (size_t)kMaxLds;
+}
+
 int fir_mfma_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
                  float *y, int64_t ldy, int precision, bool rev, hipStream_t stream,
                  DeviceState *st)

@@ -161,3 +161,21 @@ def test_segments_per_workgroup(hip, oracle, tiles, segs, K):
     dxa, _ = oracle.vjp_f64(np.abs(w), np.abs(x), np.abs(b))
     assert np.all(np.abs(dx.cpu().numpy() - dx64) <= (K + 1) * EPS * dxa)
 
+
+
+def test_tap_limit_and_auto_beyond_it(hip, oracle):
+    """BF16x3 takes K up to 6689 (the taps' tables and one tile per wave in
+    LDS); beyond it the MFMA precision is refused and AUTO takes the FP32
+    direct form, within the same bound."""
+    r = rng(31)
+    x = r.standard_normal((2, 20000)).astype(np.float32)
+    for K, mfma_ok in ((6689, True), (6690, False), (9000, False)):
+        b = (r.standard_normal(K) / np.sqrt(K)).astype(np.float32)
+        y64, bound = oracle.lfilter_f64(x, b), oracle.fir_error_bound(x, b)
+        if mfma_ok:
+            assert np.all(np.abs(host_fir(hip, x, b, "bf16x3") - y64) <= bound)
+        else:
+            with pytest.raises(hip.HipError) as e:
+                host_fir(hip, x, b, "bf16x3")
+            assert e.value.status == hip.ERR_UNSUPPORTED
+        assert np.all(np.abs(host_fir(hip, x, b, "auto") - y64) <= bound), K
