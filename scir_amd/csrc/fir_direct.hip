@@ -213,9 +213,14 @@ __global__ __launch_bounds__(kThreads) void fir_direct_dma_kernel(
         // VMEM instructions this wave issued after tile it's copies: the
         // copies of the tiles it+1 .. it+NB-2 and the stores of up to NB-1
         // earlier tiles.  Memory instructions of a wave complete in order.
+        // High wave priority from the wait to the issue of the next copies,
+        // so they leave while the other waves of the CU run FMAs (measured:
+        // +0.5-0.7 % at K = 33).
+        __builtin_amdgcn_s_setprio(3);
         wait_vmcnt(nd * max(0, min(NB - 2, ntiles - 1 - it)) + S * min(it, NB - 1));
         __syncthreads();
         if (it + NB - 1 < ntiles) issue(tfirst + it + NB - 1, (it + NB - 1) % NB);
+        __builtin_amdgcn_s_setprio(0);
         const float *buf = lds + (it % NB) * buf_chunks * 4;
         float acc[R];
         taps_from_lds<R, EXACT, true>(acc, buf, H, lane_base, K, b);
