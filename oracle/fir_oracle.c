@@ -1,9 +1,10 @@
 /*
  * CPU oracle for SciR's batched FIR path -- TEST INFRASTRUCTURE ONLY.
  *
- * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
- * this file's shared library, and only as the checker.  The product path
- * (scir_amd/, include/scir_hip.h) never links or calls it.
+ * Only tests/, __graft_entry__.smoke() and bench.py load this file's shared
+ * library: as the checker (error figures against FP64) or as the timed
+ * single-thread CPU baseline.  The product path (scir_amd/, include/scir_hip.h)
+ * never links or calls it.
  *
  * Build: gcc -O2 -std=c11 -ffp-contract=off -fno-fast-math -shared -fPIC
  * (oracle/Makefile).  -ffp-contract=off matters: rustc never contracts a*b+c

@@ -1,8 +1,9 @@
 """CPU oracle for the batched-FIR path -- TEST INFRASTRUCTURE ONLY.
 
-Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline
-leg import this module, and only as the checker; the product (``scir_amd``)
-never imports it.  It wraps ``oracle/libfir_oracle.so`` (built from
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py`` import this
+module: as the checker (bench.py's error figures against FP64) or as the timed
+single-thread CPU baseline; the product (``scir_amd``) never imports it and
+nothing measured as GPU throughput runs through it.  It wraps ``oracle/libfir_oracle.so`` (built from
 ``oracle/fir_oracle.c`` by ``oracle/Makefile``) with numpy-facing helpers.
 
 Functions and the reference lines they restate:
