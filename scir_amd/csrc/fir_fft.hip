@@ -120,6 +120,20 @@ __device__ __forceinline__ void dft(float2 (&v)[R])
 
 __device__ __forceinline__ int pad(int i) { return i + (i >> 5); }
 
+// pad(base + r * NS) - pad(base) for the store pattern of a pass with radix R
+// after NS points (base = (j / NS) * NS * R + j % NS): a compile-time offset,
+// so the LDS instructions carry it and no per-point address arithmetic is
+// left.  NS a multiple of 32: r * NS adds r * NS / 32 pads.  NS == 16 (R
+// even): base % 32 < 16, so 16r adds r / 2 pads.  NS == 1, R == 16: base is
+// 16j and r < 16 never crosses a 32-point line.
+template <int NS, int R>
+__device__ __forceinline__ constexpr int store_off(int r)
+{
+    static_assert(NS % 32 == 0 || (NS == 16 && R % 2 == 0) || (NS == 1 && R == 16),
+                  "pass shape without a constant store offset");
+    return NS % 32 == 0 ? r * (NS + NS / 32) : NS == 16 ? 16 * r + (r >> 1) : r;
+}
+
 // Twiddle tables (cos, sin of positive angles; the forward transform uses
 // the conjugates), in this order:
 //   tq[i], i < 64:        2*pi*i/NF        (fine)
@@ -146,15 +160,25 @@ __device__ __forceinline__ void fft_pass(float2 *d, const float2 *tq,
 {
     constexpr int B = NF / R;
     constexpr int PER = (B + T - 1) / T;
+    static_assert(B % 32 == 0, "loads use pad(j + r*B) == pad(j) + r*(B + B/32)");
     float2 v[PER][R];
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
         const int j = tid + p * T;
         if (B % T == 0 || j < B) {
+            const int jp = pad(j);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                v[p][r] = d[pad(j + r * B)];
-                if (MULH) v[p][r] = cmul(v[p][r], H[j + r * B]);
+                v[p][r] = d[jp + r * (B + B / 32)];
+                if (MULH) {
+                    // H[j + r*B] through a buffer resource: the per-point offset
+                    // r*B*8 is a scalar constant, so no 64-bit address arithmetic
+                    const __amdgpu_buffer_rsrc_t hr = __builtin_amdgcn_make_buffer_rsrc(
+                        const_cast<float2 *>(H), 0, NF * (int)sizeof(float2), 0x00020000);
+                    const float2 h = __builtin_bit_cast(
+                        float2, __builtin_amdgcn_raw_buffer_load_b64(hr, 8 * j, 8 * r * B, 0));
+                    v[p][r] = cmul(v[p][r], h);
+                }
             }
             if (NS > 1) {
                 const int k = j % NS;
@@ -179,9 +203,9 @@ __device__ __forceinline__ void fft_pass(float2 *d, const float2 *tq,
         const int j = tid + p * T;
         if (B % T == 0 || j < B) {
             const int k = j % NS;
-            const int base = (j / NS) * NS * R + k;
+            const int bp = pad((j / NS) * NS * R + k);
 #pragma unroll
-            for (int r = 0; r < R; ++r) d[pad(base + r * NS)] = v[p][r];
+            for (int r = 0; r < R; ++r) d[bp + store_off<NS, R>(r)] = v[p][r];
         }
     }
     __syncthreads();
@@ -265,9 +289,14 @@ __global__ __launch_bounds__(Plan<LOGN>::T) void fir_fft_kernel(
         // both windows inside their rows: no bounds checks
         const float *pa = REV ? xa + (N - 1 - wa) : xa + wa;
         const float *pb = REV ? xb + (N - 1 - wb) : xb + wb;
+        // pad(tid + m*T) == pad(tid) + m*(T + T/32): T is a multiple of 32
+        static_assert(T % 32 == 0, "constant LDS offsets need T % 32 == 0");
+        const int tp = pad(tid);
 #pragma unroll 4
-        for (int i = tid; i < NF; i += T)
-            d[pad(i)] = make_float2(pa[REV ? -i : i], pb[REV ? -i : i]);
+        for (int m = 0; m < NF / T; ++m) {
+            const int i = tid + m * T;
+            d[tp + m * (T + T / 32)] = make_float2(pa[REV ? -i : i], pb[REV ? -i : i]);
+        }
     } else {
         for (int i = tid; i < NF; i += T) {
             const int64_t ta = wa + i, tb = wb + i;
