@@ -307,10 +307,11 @@ def run_dense(args):
         for other in ("f32", "fp16", "bf16x3", "fft32"):
             if other == prec or (other == "fft32" and K > 8192):
                 continue
-            # 3 untimed calls: the first FFT32 call on a device also builds its
-            # twiddle tables, and the clock settles after the switch of kernel.
+            # 20 untimed calls: the first FFT32 call on a device also builds its
+            # twiddle tables, and the shader clock takes ~20 ms to settle after
+            # the switch from another kernel (measured after the FP16 runs).
             ms_o, _ = timed(lambda: device.fir(x, b, other, out=y), max(5, args.steps // 2),
-                            3, stream)
+                            20, stream)
             comp["fp32_direct" if other == "f32" else other] = dict(
                 gsamples_s=round(B * N / (ms_o * 1e-3) / 1e9, 2), ms_per_step=round(ms_o, 4),
                 rel_l2_vs_fp64=rel_l2_rows(x, y, b_np, rows))
