@@ -26,7 +26,8 @@ after the timed region on the same inputs:
                                       rows of the same workload (~10 s)
 
 Other workloads (``--workload``): longtap_k2048, direct_k33, direct_k129,
-ragged (10000 CSR signals), vjp_k255 (forward + dL/dx + dL/db), resample_2_3
+longtap_k16384 (x [256, 262144], 16384-tap band-pass, FFT32 in two 8192-tap
+blocks, compared with the FP32 direct form), ragged (10000 CSR signals), vjp_k255 (forward + dL/dx + dL/db), resample_2_3
 (float64 polyphase resampler, x [4096, 65536], up 2, down 3, the reference's
 31-tap filter), sosfilt_butter6 (float64 IIR cascade, x [4096, 65536], the
 reference's sosfilt benchmark filter butter(6, 0.15): 3 sections; filtfilt with
@@ -63,7 +64,12 @@ DENSE = {
                        seed=0x5C120001, outliers=0.0),
     "direct_k129": dict(B=4096, N=65536, K=129, taps="lowpass", precision="f32",
                         seed=0x5C120001, outliers=0.0),
+    # a filter far longer than the MFMA kernel takes: FFT32 in two 8192-tap blocks
+    "longtap_k16384": dict(B=256, N=262144, K=16384, taps="bandpass", precision="fft32",
+                           seed=0x5C120002, outliers=0.01),
 }
+# Longest filters the MFMA precisions take (LDS of one CU).
+MFMA_MAX_K = {"bf16x3": 6689, "fp16": 23457}
 HEADLINE = "longtap_k512"
 
 
@@ -291,7 +297,7 @@ def run_dense(args):
     stream = torch.cuda.current_stream()
     ms, wall = timed(lambda: device.fir(x, b, prec, out=y), args.steps, args.warmup, stream)
     value = B * N / (ms * 1e-3) / 1e9
-    rows = [0, B // 2, B - 1]
+    rows = [0, B // 2, B - 1] if K <= 4096 else [0, B - 1]
     out = dict(metric=METRIC, value=round(value, 2), unit="Gsamples/s", n_gpus=1,
                steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 4),
                higher_is_better=True, scaling="weak", vs_baseline=None, dtype=prec,
@@ -305,7 +311,7 @@ def run_dense(args):
     if not args.quick:
         comp = {}
         for other in ("f32", "fp16", "bf16x3", "fft32"):
-            if other == prec or (other == "fft32" and K > 8192):
+            if other == prec or K > MFMA_MAX_K.get(other, K):
                 continue
             # 20 untimed calls: the first FFT32 call on a device also builds its
             # twiddle tables, and the shader clock takes ~20 ms to settle after
