@@ -468,8 +468,10 @@ def run_vjp(args):
                 wall_s=round(wall, 3),
                 roofline=dict(bound="fp32", achieved=round(ach, 2), peak=FP32_PEAK_TFLOPS,
                               unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 4),
-                              kernel="fir_vjp_db_partial_kernel",
-                              traffic=load_traffic(f"fir_vjp_db_{prec}_vjp_k255_pmc")))
+                              kernel="fir_vjp_db_mfma_kernel (FP32 MFMA)",
+                              traffic=load_traffic(f"fir_vjp_db_{prec}_vjp_k255_pmc"),
+                              mfma_busy_frac=load_profile(f"fir_vjp_db_{prec}_vjp_k255_util",
+                                                          "mfma_busy_frac")))
 
 
 def run_resample(args):

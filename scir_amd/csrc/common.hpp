@@ -138,6 +138,7 @@ struct TuneChoice {
     // AUTO may take the FFT path (normwise error) from auto_fft_min_k up,
     // when enabled with the tuning key auto.fft = 1 (off by default).
     bool auto_fft = false;
+    bool vjp_db_mfma = true;  // dL/db on FP32 MFMA (false: the FP32 VALU kernel)
     int auto_fft_min_k = 512; // measured: FFT32 beats BF16x3 MFMA from here (K <= 8192)
 };
 TuneChoice tune_lookup(int K, int64_t N);

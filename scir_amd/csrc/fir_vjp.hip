@@ -8,7 +8,9 @@
 // N-1-i), so it reuses the dense direct/MFMA kernels through their `rev` mode
 // and inherits every precision mode.
 //
-// db is a correlation reduced over all rows and samples.  Stage 1: a
+// db is a correlation reduced over all rows and samples.  Stage 1 runs on
+// FP32 MFMA by default (fir_vjp_db_mfma_kernel, below); the tuning key
+// "vjp.db_mfma" = 0 selects the FP32 VALU form: a
 // workgroup walks `tiles_per_wg` tiles of T = 2048 samples of one row,
 // staging g and x (with K samples of history) in LDS.  Lane (kg, tg) owns 16
 // lags k0 + kg*16 + (0..15) and 128 samples tg*128 + (0..127) of a tile; it
@@ -123,33 +125,143 @@ __global__ __launch_bounds__(kThreads) void fir_vjp_db_partial_kernel(
     }
 }
 
+// The same stage 1 on FP32 MFMA (v_mfma_f32_16x16x4_f32).  With lags
+// k = kp + i + 16j (i, j < 16) one 16x16x4 product takes four samples m..m+3
+// against all 256 lags of a pass:
+//   C[i][j] += sum_l x[m + l - i - kp] * g[m + l + 16j],
+// A[i][l] = x[m + l - i - kp] (a Toeplitz slice of x), B[l][j] =
+// g[m + l + 16j]; summed over every m of the row, C[i][j] is
+// db[kp + i + 16j] (FP32 operands, FP32 accumulation; no product is wasted
+// for K <= 256).  A tile stages g[t0, t0 + 2304) and x[t0 - kp - 16,
+// t0 - kp + 2064) in LDS with 4 pad floats per 16, which puts the 64
+// B-operand reads of a product (16 lanes 16 samples apart) in distinct
+// banks; the next tile's samples are loaded into registers while the current
+// tile's products run.  Wave w takes the tile's samples 512w .. 512w + 511
+// with four accumulators in turn (32 products each per tile, as long as the
+// VALU kernel's per-tile chains); tile sums, the four waves' sums (fixed
+// order) and partial[wg][k] follow, which the final kernel reduces as before.
+constexpr int kMfmaLook = 256;  // g samples staged past the tile (16j + l < 256)
+
+__global__ __launch_bounds__(kThreads) void fir_vjp_db_mfma_kernel(
+    const float *__restrict__ g, int64_t ldg, const float *__restrict__ x, int64_t ldx, int K,
+    int64_t N, int segs_per_row, int tiles_per_wg, float *__restrict__ partial, int vec_g,
+    int vec_x)
+{
+    constexpr int GS = kTile + kMfmaLook;  // staged g samples
+    constexpr int XS = kTile + 32;         // staged x samples
+    __shared__ __attribute__((aligned(16))) float gs[GS + GS / 4];
+    __shared__ __attribute__((aligned(16))) float xs[XS + XS / 4];
+    __shared__ float red[kThreads / 64][kLagPass];
+
+    const int64_t row = blockIdx.x / segs_per_row;
+    const int64_t tile0 = (int64_t)(blockIdx.x % segs_per_row) * tiles_per_wg;
+    const float *__restrict__ grow = g + row * ldg;
+    const float *__restrict__ xrow = x + row * ldx;
+    const int64_t tiles_in_row = (N + kTile - 1) / kTile;
+    const int ntiles = (int)min<int64_t>(tiles_per_wg, tiles_in_row - tile0);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lq = lane >> 4;  // A: (row li, col lq); B: (row lq, col li)
+    const int a_off = 16 + lq - li;  // x sample m + lq - li - kp, staged from t0 - kp - 16
+    const int b_off = lq + 16 * li;  // g sample m + lq + 16 li, staged from t0
+
+    constexpr int NG = (GS / 4 + kThreads - 1) / kThreads, NX = (XS / 4 + kThreads - 1) / kThreads;
+    f32x4 pg[NG], px[NX];
+    auto fetch = [&](int64_t t0, int kp) {
+#pragma unroll
+        for (int u = 0; u < NG; ++u) {
+            const int v = threadIdx.x + u * kThreads;
+            if (v < GS / 4) pg[u] = load4g(grow, t0 + 4 * v, N, vec_g);
+        }
+#pragma unroll
+        for (int u = 0; u < NX; ++u) {
+            const int v = threadIdx.x + u * kThreads;
+            if (v < XS / 4) px[u] = load4g(xrow, t0 - kp - 16 + 4 * v, N, vec_x);
+        }
+    };
+    for (int kp = 0; kp < K; kp += kLagPass) {
+        f32x4 tot = {0.0f, 0.0f, 0.0f, 0.0f};
+        fetch(tile0 * kTile, kp);
+        for (int it = 0; it < ntiles; ++it) {
+            __syncthreads();  // previous tile done with gs/xs
+#pragma unroll
+            for (int u = 0; u < NG; ++u) {
+                const int v = threadIdx.x + u * kThreads;
+                if (v < GS / 4) *reinterpret_cast<f32x4 *>(gs + pad16(4 * v)) = pg[u];
+            }
+#pragma unroll
+            for (int u = 0; u < NX; ++u) {
+                const int v = threadIdx.x + u * kThreads;
+                if (v < XS / 4) *reinterpret_cast<f32x4 *>(xs + pad16(4 * v)) = px[u];
+            }
+            __syncthreads();
+            if (it + 1 < ntiles) fetch((tile0 + it + 1) * kTile, kp);
+            // Four accumulators in turn (independent MFMA chains of 32 products
+            // of 4 samples each per tile), added into the pass total per tile.
+            f32x4 acc[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 4
+            for (int q = 0; q < kTile / 4 / 4; q += 4) {  // 128 groups of 4 samples per wave
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int m = 512 * wave + 4 * (q + u);
+                    acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                        xs[pad16(m + a_off)], gs[pad16(m + b_off)], acc[u], 0, 0, 0);
+                }
+            }
+            tot += (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        }
+        // C[i][j] in lane (j = li, i = 4 lq + v): lag kp + 4 lq + v + 16 li.
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[wave][4 * lq + v + 16 * li] = tot[v];
+        __syncthreads();
+        {
+            const int lag = kp + threadIdx.x;
+            const float sum = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) +
+                              red[3][threadIdx.x];
+            if (lag < K) partial[(int64_t)blockIdx.x * K + lag] = sum;
+        }
+        __syncthreads();  // red is rewritten by the next pass
+    }
+}
+
+constexpr int kFinalLags = 4;                       // lags per final workgroup
+constexpr int kFinalSplit = kThreads / kFinalLags;  // workgroup residues per lag
+
+__device__ __forceinline__ void kahan_add(float &s, float &c, float v)
+{
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
 // db[k] = sum_w partial[w][k], Kahan-compensated, fixed order: lane (lag, q)
-// sums workgroups w == q (mod 16), then the 16 sums are added in q order.
+// sums workgroups w == q (mod 64) in increasing w (loads issued eight at a
+// time), then the 64 sums are added in q order.
 __global__ __launch_bounds__(kThreads) void fir_vjp_db_final_kernel(
     const float *__restrict__ partial, int64_t nwg, int K, float *__restrict__ db)
 {
-    __shared__ float sums[16][17];
-    const int lag_l = threadIdx.x & 15, q = threadIdx.x >> 4;
-    const int lag = blockIdx.x * 16 + lag_l;
+    __shared__ float sums[kFinalSplit][kFinalLags + 1];
+    const int lag_l = threadIdx.x % kFinalLags, q = threadIdx.x / kFinalLags;
+    const int lag = blockIdx.x * kFinalLags + lag_l;
     float s = 0.0f, c = 0.0f;
     if (lag < K) {
-        for (int64_t w = q; w < nwg; w += 16) {
-            const float y = partial[w * K + lag] - c;
-            const float t = s + y;
-            c = (t - s) - y;
-            s = t;
+        int64_t w = q;
+        for (; w + 7 * kFinalSplit < nwg; w += 8 * kFinalSplit) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = partial[(w + u * kFinalSplit) * K + lag];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) kahan_add(s, c, v[u]);
         }
+        for (; w < nwg; w += kFinalSplit) kahan_add(s, c, partial[w * K + lag]);
     }
     sums[q][lag_l] = s;
     __syncthreads();
     if (q == 0 && lag < K) {
         float tot = 0.0f, cc = 0.0f;
-        for (int j = 0; j < 16; ++j) {
-            const float y = sums[j][lag_l] - cc;
-            const float t = tot + y;
-            cc = (t - tot) - y;
-            tot = t;
-        }
+        for (int j = 0; j < kFinalSplit; ++j) kahan_add(tot, cc, sums[j][lag_l]);
         db[lag] = tot;
     }
 }
@@ -181,8 +293,9 @@ int vjp_db_run(const float *g, int64_t ldg, const float *x, int64_t ldx, int32_t
 {
     if (B == 0 || N == 0) {
         // Empty sum: db = 0 (a one-row, zero-work pass of the final kernel).
-        hipLaunchKernelGGL(fir_vjp_db_final_kernel, dim3((K + 15) / 16), dim3(kThreads), 0, s,
-                           static_cast<const float *>(nullptr), (int64_t)0, (int)K, db);
+        hipLaunchKernelGGL(fir_vjp_db_final_kernel, dim3((K + kFinalLags - 1) / kFinalLags),
+                           dim3(kThreads), 0, s, static_cast<const float *>(nullptr), (int64_t)0,
+                           (int)K, db);
         return check_launch("fir_vjp_db_final_kernel");
     }
     const int64_t tiles_in_row = (N + kTile - 1) / kTile;
@@ -194,6 +307,19 @@ int vjp_db_run(const float *g, int64_t ldg, const float *x, int64_t ldx, int32_t
     segs = (tiles_in_row + tiles_per_wg - 1) / tiles_per_wg;
     const int64_t nwg = B * segs;
     if (nwg > 0x7fffffff) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "batch too large");
+    const TuneChoice tc = tune_lookup(K, N);
+    if (tc.vjp_db_mfma) {
+        StreamScratch &sc = scratch_for(st, s);
+        SCIR_TRY(ensure_buffer(sc.partials, (size_t)nwg * K * sizeof(float), s));
+        float *partial = static_cast<float *>(sc.partials.ptr);
+        hipLaunchKernelGGL(fir_vjp_db_mfma_kernel, dim3((unsigned)nwg), dim3(kThreads), 0, s, g,
+                           ldg, x, ldx, (int)K, N, (int)segs, (int)tiles_per_wg, partial,
+                           aligned16(g, ldg), aligned16(x, ldx));
+        SCIR_TRY(check_launch("fir_vjp_db_mfma_kernel"));
+        hipLaunchKernelGGL(fir_vjp_db_final_kernel, dim3((K + kFinalLags - 1) / kFinalLags),
+                           dim3(kThreads), 0, s, partial, nwg, (int)K, db);
+        return check_launch("fir_vjp_db_final_kernel");
+    }
     const int passes = (K + kLagPass - 1) / kLagPass;
     const int hist = ((passes * kLagPass + 16) + 3) & ~3;  // history samples before a tile
     const size_t lds_floats =
@@ -210,8 +336,8 @@ int vjp_db_run(const float *g, int64_t ldg, const float *x, int64_t ldx, int32_t
                        ldg, x, ldx, (int)K, N, (int)segs, (int)tiles_per_wg, hist, partial,
                        aligned16(g, ldg), aligned16(x, ldx));
     SCIR_TRY(check_launch("fir_vjp_db_partial_kernel"));
-    hipLaunchKernelGGL(fir_vjp_db_final_kernel, dim3((K + 15) / 16), dim3(kThreads), 0, s, partial,
-                       nwg, (int)K, db);
+    hipLaunchKernelGGL(fir_vjp_db_final_kernel, dim3((K + kFinalLags - 1) / kFinalLags),
+                       dim3(kThreads), 0, s, partial, nwg, (int)K, db);
     return check_launch("fir_vjp_db_final_kernel");
 }
 
