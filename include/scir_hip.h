@@ -303,6 +303,7 @@ SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
  *   "mfma.tiles"      32x32 output tiles per wave of the MFMA kernel (1, 2, 4)
  *   "mfma.segs"       row segments per workgroup of the MFMA kernel (>= 1); the
  *                     next segment's samples load while the current one computes
+ *   "vjp.db_mfma"     1 (default): dL/db on FP32 MFMA; 0: the FP32 VALU kernel
  *   "auto.mfma_min_k" smallest K for which AUTO picks BF16x3 MFMA
  *   "auto.fft"        1: AUTO may pick the FFT32 path (normwise error) from
  *                     auto.fft_min_k up (x != y); 0 (default): never
