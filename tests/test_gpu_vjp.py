@@ -58,6 +58,27 @@ def test_grads_vs_fp64(hip, oracle, B, N, K, precision):
     assert np.all(np.abs(db - db64) <= bdb), f"db max excess {np.max(np.abs(db - db64) - bdb)}"
 
 
+@pytest.mark.parametrize("mfma", [0, 1])
+@pytest.mark.parametrize("B,N,K", [(1, 1, 1), (2, 2049, 255), (1, 9000, 300), (2, 5000, 600),
+                                   (3, 70000, 255)])
+def test_db_forms_vs_fp64(hip, oracle, B, N, K, mfma):
+    """Both stage-1 forms of dL/db (FP32 MFMA, FP32 VALU) within the bound,
+    including K beyond one 256-lag pass and rows of several workgroups."""
+    L = hip.lib()
+    hip.check(L.scir_hip_tune_set(b"vjp.db_mfma", mfma))
+    try:
+        r = rng(B * 11 + N + K + mfma)
+        x = r.standard_normal((B, N)).astype(np.float32)
+        b = r.standard_normal(K).astype(np.float32)
+        w = r.standard_normal((B, N)).astype(np.float32)
+        _, db = gpu_grads(x, b, w)
+        _, db64 = oracle.vjp_f64(w, x, b)
+        _, bdb = bounds(oracle, x, b, w)
+        assert np.all(np.abs(db - db64) <= bdb), f"db max excess {np.max(np.abs(db - db64) - bdb)}"
+    finally:
+        hip.check(L.scir_hip_tune_set(b"vjp.db_mfma", -1))
+
+
 def test_finite_differences(hip, oracle):
     """64 seeded coordinates (x and b), h = 1e-3, FP64 loss via the oracle."""
     r = rng(64)
