@@ -197,19 +197,23 @@ __global__ __launch_bounds__(kThreads) void fir_vjp_db_mfma_kernel(
             if (it + 1 < ntiles) fetch((tile0 + it + 1) * kTile, kp);
             // Four accumulators in turn (independent MFMA chains of 32 products
             // of 4 samples each per tile), added into the pass total per tile.
-            f32x4 acc[4];
+            // Sample group 4(4t + u) of the wave: each 16-sample step moves a
+            // padded index by exactly 20 floats, so after the four bases the
+            // fully unrolled loop's LDS reads carry constant offsets.
+            const int m0 = 512 * wave;
+            const float *xa0 = xs + pad16(m0 + a_off), *xa1 = xs + pad16(m0 + 4 + a_off);
+            const float *xa2 = xs + pad16(m0 + 8 + a_off), *xa3 = xs + pad16(m0 + 12 + a_off);
+            const float *gb0 = gs + pad16(m0 + b_off), *gb1 = gs + pad16(m0 + 4 + b_off);
+            const float *gb2 = gs + pad16(m0 + 8 + b_off), *gb3 = gs + pad16(m0 + 12 + b_off);
+            f32x4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = c0, c2 = c0, c3 = c0;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) acc[u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll 4
-            for (int q = 0; q < kTile / 4 / 4; q += 4) {  // 128 groups of 4 samples per wave
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int m = 512 * wave + 4 * (q + u);
-                    acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                        xs[pad16(m + a_off)], gs[pad16(m + b_off)], acc[u], 0, 0, 0);
-                }
+            for (int t = 0; t < kTile / 4 / 16; ++t) {  // 32 steps of 16 samples per wave
+                c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa0[20 * t], gb0[20 * t], c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa1[20 * t], gb1[20 * t], c1, 0, 0, 0);
+                c2 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa2[20 * t], gb2[20 * t], c2, 0, 0, 0);
+                c3 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa3[20 * t], gb3[20 * t], c3, 0, 0, 0);
             }
-            tot += (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            tot += (c0 + c1) + (c2 + c3);
         }
         // C[i][j] in lane (j = li, i = 4 lq + v): lag kp + 4 lq + v + 16 li.
 #pragma unroll
