@@ -226,8 +226,10 @@ SCIR_HIP_API int scir_hip_filtfilt_f64_cpu(const double *x, int64_t ldx, int64_t
  * same layout as x.  A plan holds the host-built, cost-sorted work list and
  * the device work queue; executing it is one persistent launch.  Executing a
  * plan does not allocate, so it may be captured in a HIP graph.
- * precision: F32 / AUTO (FP32 direct form), F32_EXACT (the reference loop's
- * order, bitwise), or BF16X3 / FP16 (the MFMA Toeplitz contraction with the
+ * precision: F32 (FP32 direct form), F32_EXACT (the reference loop's order,
+ * bitwise), AUTO (resolved when the plan is made: BF16X3 when the batch's
+ * multiply-add weighted tap count reaches the tuned MFMA cut-over and the
+ * longest filter fits, F32 otherwise), or BF16X3 / FP16 (the MFMA Toeplitz contraction with the
  * dense MFMA kernel's arithmetic, segments of 8192 outputs; taps up to the LDS
  * limit: K <= 5601 for BF16X3).  FFT32 gives SCIR_HIP_ERR_UNSUPPORTED. */
 typedef struct scir_hip_ragged_plan_s *scir_hip_ragged_plan;

@@ -155,6 +155,31 @@ int scir_hip_ragged_plan_create(const int64_t *offs, const int64_t *b_offs, int6
     if (!plan) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "plan is null");
     *plan = nullptr;
     SCIR_TRY(validate_csr(offs, b_offs, S));
+    if (precision == SCIR_HIP_PREC_AUTO) {
+        // As for dense batches: BF16x3 MFMA when the batch's taps (mean K
+        // weighted by multiply-adds) reach the tuned cut-over and its LDS holds
+        // the longest filter; the FP32 direct form otherwise.  Both keep the
+        // elementwise FP32 bound.
+        double macs = 0.0, kmacs = 0.0, nmean = 0.0;
+        int kmax_all = 0;
+        for (int64_t s = 0; s < S; ++s) {
+            const double len = (double)(offs[s + 1] - offs[s]);
+            const double K = (double)(b_offs[s + 1] - b_offs[s]);
+            macs += len * K;
+            kmacs += len * K * K;
+            nmean += len;
+            if (len > 0) kmax_all = std::max(kmax_all, (int)(b_offs[s + 1] - b_offs[s]));
+        }
+        const int kmean = macs > 0.0 ? (int)(kmacs / macs) : 0;
+        size_t lds_need = 0;
+        const bool fits = kmax_all > 0 &&
+                          fir_mfma_ragged_lds(SCIR_HIP_PREC_BF16X3, kmax_all, &lds_need) ==
+                              SCIR_HIP_OK;
+        clear_error();
+        const TuneChoice tc = tune_lookup(std::max(kmean, 1), S > 0 ? (int64_t)(nmean / S) : 0);
+        precision = (fits && kmean >= tc.auto_mfma_min_k) ? SCIR_HIP_PREC_BF16X3
+                                                           : SCIR_HIP_PREC_F32;
+    }
     RaggedMode mode = kDirect;
     SCIR_TRY(ragged_mode(precision, &mode));
     DeviceState *st = nullptr;

@@ -103,6 +103,19 @@ def test_mfma_ragged_rel_l2(hip, oracle, precision, tol):
     assert oracle.rel_l2(y, y64) <= tol
 
 
+def test_auto_precision_resolves_per_plan(hip, oracle):
+    """AUTO on a ragged plan: BF16x3 MFMA when the batch's multiply-add
+    weighted K reaches the cut-over (output equal to an explicit bf16x3
+    plan), the FP32 direct form for short filters (equal to an f32 plan);
+    both within the elementwise FP32 bound."""
+    for seed, max_k, expect in ((17, 900, "bf16x3"), (18, 20, "f32")):
+        offs, boffs, x, b = make_case(seed, 30, 20000, max_k, allow_empty=False)
+        y = run(offs, boffs, x, b, "auto")
+        np.testing.assert_array_equal(y, run(offs, boffs, x, b, expect))
+        y64 = oracle.lfilter_ragged_f64(x, offs, b, boffs)
+        assert np.all(np.abs(y - y64) <= ragged_bound(oracle, x, offs, b, boffs)), expect
+
+
 def test_mfma_ragged_unaligned_signals(hip, oracle):
     """Signals starting at odd sample offsets take the scalar load/store path."""
     offs = np.array([0, 3, 9004, 9005, 30001], dtype=np.int64)
