@@ -13,7 +13,20 @@
  *   - All functions return an int status (scir_hip_status); 0 is success.  On
  *     failure scir_hip_last_error() returns a message naming the failing call
  *     (the thread's last error), like GpuError::BackendUnavailable(String)
- *     (crates/scir-gpu/src/lib.rs:583-589).  No C++ exception crosses the ABI.
+ *     (crates/scir-gpu/src/lib.rs:583-589).  No C++ exception crosses the ABI:
+ *     a failed host allocation inside the library returns
+ *     SCIR_HIP_ERR_OUT_OF_MEMORY, any other internal exception
+ *     SCIR_HIP_ERR_BACKEND.  For tests of these paths, every allocation the
+ *     library makes for itself fails while the environment variable
+ *     SCIR_HIP_FAIL_ALLOC is set to a non-zero integer.
+ *   - Non-finite values: every precision gives a non-finite output exactly
+ *     where the reference loop (crates/scir-gpu/src/lib.rs:1141-1148, which
+ *     forms taps[K-1-t] * x[i-t] for t <= min(i, K-1) only) gives one, with
+ *     the same infinities; F32_EXACT equals it bit for bit, NaN for NaN.  The
+ *     kernels stage zeros before the row start (direct form), multiply zero
+ *     taps of Toeplitz blocks (MFMA) or transform whole segments (FFT), which
+ *     spreads an Inf or NaN to neighbouring outputs; each output that comes
+ *     out non-finite is recomputed from the reference loop's terms.
  *   - Taps are passed as `b` in lfilter order: y[n] = sum_k b[k] * x[n-k] with
  *     x[<0] = 0 (zero initial state), output length == input length.  The
  *     reference's `taps` argument is b reversed (b[k] == taps[K-1-k]); the
@@ -66,15 +79,20 @@ typedef enum scir_hip_precision {
      * +0.0f): bit-identical to crates/scir-gpu/src/lib.rs:1134-1152. */
     SCIR_HIP_PREC_F32_EXACT = 1,
     /* Toeplitz-block contraction on MFMA: x and b split into three bf16 terms
-     * (hi+mid+lo), six cross products kept, FP32 accumulation.  K <= 6689
-     * (FP16: K <= 23457); longer filters give SCIR_HIP_ERR_UNSUPPORTED. */
+     * (hi+mid+lo), six cross products kept, FP32 accumulation; within
+     * (K+1) * 2^-23 * sum_k |b_k||x_{n-k}| of the exact sum like F32.
+     * K <= 6689 (FP16: K <= 23457); longer filters give
+     * SCIR_HIP_ERR_UNSUPPORTED.  Values beyond the bf16 (fp16) range and
+     * non-finite values take the recomputation described above. */
     SCIR_HIP_PREC_BF16X3 = 2,
     /* Toeplitz-block contraction on MFMA with one fp16 product (x and b
      * rounded to fp16), FP32 accumulation.  Reported, not accuracy-bound. */
     SCIR_HIP_PREC_FP16 = 3,
     /* Fastest of F32 / BF16X3 for the shape (autotuned table when loaded,
      * else built-in cut-over; F32 for K > 6689, beyond the MFMA kernel's
-     * LDS); never FP16, and FFT32 only when the "auto.fft" key is set. */
+     * LDS); never FP16, and FFT32 only when the "auto.fft" key is set.  Either
+     * choice keeps the F32 bound and the reference loop's non-finite
+     * outputs, so AUTO is the drop-in default. */
     SCIR_HIP_PREC_AUTO = 4,
     /* FP32 FFT overlap-save: cost O(log K) per output instead of O(K) for
      * K <= 8192; longer filters run as blocks of 8192 taps (partitioned
@@ -86,13 +104,14 @@ typedef enum scir_hip_precision {
 } scir_hip_precision;
 
 /* ---- runtime ------------------------------------------------------------ */
-/* Version of this ABI (major*10000 + minor*100 + patch). */
+/* Version of this ABI (major*10000 + minor*100 + patch): 10100. */
 SCIR_HIP_API int scir_hip_version(void);
 /* Initialise the runtime on the current HIP device (idempotent).  Replaces the
  * per-call CudaCtx::create_default + cuModuleLoadData of
  * crates/scir-gpu/src/lib.rs:601-622, 814-824: state is created once and kept. */
 SCIR_HIP_API int scir_hip_init(void);
-/* Release library-owned buffers on every device (idempotent). */
+/* Release library-owned buffers on every device (idempotent).  HIP graphs
+ * that captured library calls must be destroyed first. */
 SCIR_HIP_API int scir_hip_shutdown(void);
 /* Number of visible HIP devices (0 when none), like cuda_available
  * (crates/scir-gpu/src/lib.rs:591-599); never fails for a missing device. */
@@ -229,9 +248,16 @@ SCIR_HIP_API int scir_hip_filtfilt_f64_cpu(const double *x, int64_t ldx, int64_t
  * precision: F32 (FP32 direct form), F32_EXACT (the reference loop's order,
  * bitwise), AUTO (resolved when the plan is made: BF16X3 when the batch's
  * multiply-add weighted tap count reaches the tuned MFMA cut-over and the
- * longest filter fits, F32 otherwise), or BF16X3 / FP16 (the MFMA Toeplitz contraction with the
- * dense MFMA kernel's arithmetic, segments of 8192 outputs; taps up to the LDS
- * limit: K <= 5601 for BF16X3).  FFT32 gives SCIR_HIP_ERR_UNSUPPORTED. */
+ * longest filter fits, F32 otherwise), or BF16X3 / FP16 (the MFMA Toeplitz
+ * contraction with the dense MFMA kernel's arithmetic, segments of 8192
+ * outputs; taps up to the LDS limit: K <= 5601 for BF16X3).  FFT32 gives
+ * SCIR_HIP_ERR_UNSUPPORTED.  Launch shapes (persistent workgroups per CU,
+ * work per item) come from the tuning keys ragged.* when the plan is made.
+ * Each plan owns one device work-queue counter: at most one execution of a
+ * plan may be in flight at a time (executions on one stream are ordered;
+ * on several streams, or a graph replay beside an eager execute, they race
+ * and lose or repeat items), and a plan must outlive every graph that
+ * captured its execution. */
 typedef struct scir_hip_ragged_plan_s *scir_hip_ragged_plan;
 /* offs[S+1], b_offs[S+1]: HOST arrays (int64, non-decreasing, offs[0] ==
  * b_offs[0] == 0). */
@@ -277,7 +303,13 @@ SCIR_HIP_API int scir_hip_fir_vjp_f32_host(const float *x, int64_t ldx, const fl
  * Capture any sequence of *_dev / ragged-plan calls enqueued on `stream`
  * between begin and end, then replay it.  Calls that would need to grow
  * library scratch during capture fail with SCIR_HIP_ERR_CAPTURE: run the same
- * call once eagerly first. */
+ * call once eagerly first.  A graph keeps the addresses of the scratch its
+ * calls used (the dL/db partials, the FFT filter spectrum, the IIR chunk
+ * states of the stream): when a later eager call on that stream needs more
+ * scratch, the library allocates a larger block and keeps the old one
+ * until scir_hip_shutdown, so the graph stays valid.  A replay and an eager
+ * call on the same stream are ordered; calls on other streams use their own
+ * scratch. */
 typedef struct scir_hip_graph_s *scir_hip_graph;
 SCIR_HIP_API int scir_hip_graph_begin(void *stream);
 SCIR_HIP_API int scir_hip_graph_end(void *stream, scir_hip_graph *graph);
@@ -312,7 +344,16 @@ SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
  *   "auto.fft_min_k"  smallest K for which AUTO (with auto.fft) picks FFT32;
  *                     the autotuner measures it (the K from which FFT32 beats
  *                     BF16x3 MFMA on long rows)
- * The autotuner's search uses the same mechanism. */
+ *   "ragged.direct_wg_per_cu"      persistent workgroups per CU of ragged
+ *                                  batches in the FP32 direct form (1 .. 32)
+ *   "ragged.direct_min_item_kmacs" multiply-adds per direct-form work item,
+ *                                  at least, in units of 1024
+ *   "ragged.mfma_segs_per_item"    8192-output segments per MFMA work item,
+ *                                  at most (1 .. 1024)
+ *   "ragged.mfma_wg_per_cu"        persistent workgroups per CU of ragged
+ *                                  MFMA batches (0: as many as fit, 1 .. 8)
+ * The ragged keys apply to plans made after the call.  The autotuner's
+ * search uses the same mechanism. */
 SCIR_HIP_API int scir_hip_tune_set(const char *key, int32_t value);
 
 /* ---- synthetic inputs -------------------------------------------------------

@@ -1,8 +1,9 @@
 """Build libscir_hip.so in-tree (scir_amd/libscir_hip.so) for gfx950.
 
-Every ``*.hip`` under scir_amd/csrc is compiled by hipcc for gfx950 only; the
-CPU arm (cpu_fir.cpp) is compiled by g++ with FP contraction off, since it
-must keep the reference's separately rounded multiply and add.  Objects go
+Every ``*.hip`` under scir_amd/csrc is compiled by hipcc for gfx950 only;
+the host-only ``*.cpp`` files (the CPU arms, error reporting, the tuning
+table parser, the ragged plan builder) by g++ with FP contraction off, since
+the CPU arms must keep the reference's separately rounded multiply and add.  Objects go
 to scir_amd/_build/ and the shared library is linked with hipcc.
 
 Run as ``python -m scir_amd.build`` or through ``__graft_entry__.build()``.
@@ -61,14 +62,9 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
         obj = os.path.join(BUILD, os.path.basename(src) + ".o")
         cmds.append([HIPCC, *HIP_FLAGS, "-c", src, "-o", obj])
         objs.append(obj)
-    for src in cpps:
+    for src in cpps:  # host-only code (no HIP headers): g++
         obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-        if os.path.basename(src) == "cpu_fir.cpp":
-            cmds.append([CXX, *CXX_FLAGS, "-c", src, "-o", obj])
-        else:
-            cmds.append([HIPCC, *[f for f in HIP_FLAGS if not f.startswith("--offload-arch")],
-                         "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-x", "c++", "-c", src,
-                         "-o", obj])
+        cmds.append([CXX, *CXX_FLAGS, "-c", src, "-o", obj])
         objs.append(obj)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         for out in ex.map(_run, cmds):

@@ -10,15 +10,11 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <vector>
 
-#include "../../include/scir_hip.h"
+#include "host_common.hpp"
 
 namespace scir {
-
-// Records `code` and a printf-style message as this thread's last error and
-// returns `code`, so call sites read `return fail(...)`.
-int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
-void clear_error();
 
 #define SCIR_HIP_TRY(call)                                                            \
     do {                                                                              \
@@ -27,13 +23,6 @@ void clear_error();
             return ::scir::fail(e_ == hipErrorOutOfMemory ? SCIR_HIP_ERR_OUT_OF_MEMORY \
                                                           : SCIR_HIP_ERR_BACKEND,     \
                                 "%s failed: %s", #call, hipGetErrorString(e_));       \
-    } while (0)
-
-#define SCIR_TRY(expr)              \
-    do {                            \
-        int s_ = (expr);            \
-        if (s_ != SCIR_HIP_OK)      \
-            return s_;              \
     } while (0)
 
 // A grow-only device allocation owned by the library.
@@ -60,18 +49,38 @@ struct DeviceState {
     std::string arch;
     std::mutex mu;  // guards everything below
     std::map<void *, StreamScratch> scratch;
-    // Host tier: device copies of the caller's host arrays.
+    // Scratch blocks outgrown while a captured graph may still hold their
+    // addresses: kept until scir_hip_shutdown.
+    std::vector<void *> retired;
+    DeviceBuffer fft_twiddles[3];  // FFT path tables for NF = 2^12, 2^13, 2^14
+    // Host tier: device copies of the caller's host arrays, and pinned host
+    // staging buffers for the chunked copies.
     DeviceBuffer host_x, host_y, host_b;
     DeviceBuffer host_z;  // third matrix of the host-tier adjoint (dL/dx)
+    void *pinned[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t pinned_bytes = 0;
     hipStream_t host_stream = nullptr;
+    hipStream_t host_copy_stream[2] = {nullptr, nullptr};
 };
 
 // Initialises (once) and returns the state of the current device.
 int current_device_state(DeviceState **out);
 
-// Makes `buf` hold at least `bytes`.  Growing frees the old block; a growth
-// requested while `stream` is capturing fails with SCIR_HIP_ERR_CAPTURE.
+// hipMalloc of library-owned device memory (fails while allocation-failure
+// injection is on).
+int device_alloc(void **ptr, size_t bytes, const char *what);
+
+// Makes `buf` hold at least `bytes` of a host-tier buffer (never captured):
+// growing frees the old block.
 int ensure_buffer(DeviceBuffer &buf, size_t bytes, hipStream_t stream);
+
+// Makes the scratch block `buf` of device state `st` hold at least `bytes`.
+// A HIP graph captured earlier may hold the old block's address, so growing
+// retires it (freed at scir_hip_shutdown) instead of freeing it; sizes grow
+// at least twofold, so retired blocks add up to less than the live one.  A
+// growth requested while `stream` is capturing fails with
+// SCIR_HIP_ERR_CAPTURE.  The caller holds st->mu.
+int ensure_scratch(DeviceState *st, DeviceBuffer &buf, size_t bytes, hipStream_t stream);
 
 // Scratch of `stream` on device state `st` (caller holds st->mu).
 StreamScratch &scratch_for(DeviceState *st, hipStream_t stream);
@@ -107,13 +116,6 @@ int fir_dense_dispatch(const float *x, int64_t ldx, const float *b, int32_t K, i
 int fill_zero_2d(float *y, int64_t ldy, int64_t B, int64_t N, hipStream_t stream);
 
 // ---- ragged (CSR) batches on MFMA ----------------------------------------
-// One work item: segments seg0 .. seg0+nseg-1 (8192 outputs each) of the
-// signal x[x0 .. x0+len) with taps b[b0 .. b0+K) (lfilter order).
-struct MfmaRaggedItem {
-    int64_t x0, len, b0;
-    int32_t K, seg0, nseg, pad;
-};
-constexpr int kMfmaRaggedSegLen = 8192;  // outputs per segment (4 waves x 2 tiles x 32 x 32)
 // Dynamic LDS bytes of the ragged MFMA kernel for taps up to kmax; fails with
 // SCIR_HIP_ERR_UNSUPPORTED when that exceeds the CU's LDS.
 int fir_mfma_ragged_lds(int precision, int kmax, size_t *lds);
@@ -124,23 +126,7 @@ int fir_mfma_ragged_launch(const float *x, const float *b, float *y, const MfmaR
                            int n_items, unsigned int *counter, int precision, int kmax,
                            size_t lds, int grid, hipStream_t stream);
 
-// Autotuned choices (defaults until a table is loaded or a search ran).
-struct TuneChoice {
-    int direct_r = 8;         // outputs per lane in the direct kernel
-    int direct_loader = 2;    // 0: one tile per workgroup, 1: register prefetch, 2: LDS DMA
-    int direct_seg_tiles = 4; // tiles per workgroup of the pipelined variants
-    int direct_dma_bufs = 2;  // LDS buffers of the DMA loader (tiles in flight + 1), 2..4
-    int mfma_tiles = 2;       // 32x32 output tiles per wave, BF16x3 MFMA kernel
-    int mfma_tiles_fp16 = 4;  // the same for the FP16 MFMA kernel
-    int mfma_segs = 16;       // row segments per workgroup of the MFMA kernel
-    bool mfma_segs_forced = false;  // set through scir_hip_tune_set: no grid-size cap
-    int auto_mfma_min_k = 56; // AUTO uses BF16x3 MFMA from this K up
-    // AUTO may take the FFT path (normwise error) from auto_fft_min_k up,
-    // when enabled with the tuning key auto.fft = 1 (off by default).
-    bool auto_fft = false;
-    bool vjp_db_mfma = true;  // dL/db on FP32 MFMA (false: the FP32 VALU kernel)
-    int auto_fft_min_k = 512; // measured: FFT32 beats BF16x3 MFMA from here (K <= 8192)
-};
 TuneChoice tune_lookup(int K, int64_t N);
+RaggedTune ragged_tune_lookup();
 
 }  // namespace scir

@@ -13,12 +13,32 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/scir_hip.h"
+#include "host_common.hpp"
 
-namespace scir {
-int fail(int code, const char *fmt, ...);
-void clear_error();
-}  // namespace scir
+using scir::guarded;
+
+namespace {
+
+// Runs work(0) .. work(nt-1) on nt threads (work(0) on the caller's).  If a
+// thread cannot be started, the ones already running are joined before the
+// exception propagates (a joinable std::thread must never be destroyed).
+template <class F>
+void run_threads(int nt, F &work)
+{
+    scir::host_alloc_point();
+    std::vector<std::thread> pool;
+    pool.reserve(nt);
+    try {
+        for (int w = 1; w < nt; ++w) pool.emplace_back(work, w);
+    } catch (...) {
+        for (auto &t : pool) t.join();
+        throw;
+    }
+    work(0);
+    for (auto &t : pool) t.join();
+}
+
+}  // namespace
 
 namespace {
 
@@ -55,32 +75,31 @@ extern "C" int scir_hip_fir_f32_cpu(const float *x, int64_t ldx, const float *b,
                                     int64_t B, int64_t N, float *y, int64_t ldy,
                                     int32_t nthreads)
 {
-    scir::clear_error();
-    if (B < 0 || N < 0 || K < 0)
-        return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "negative size");
-    if (B == 0 || N == 0) return SCIR_HIP_OK;
-    if (!x || !y || (K > 0 && !b))
-        return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "null pointer argument");
-    if (ldx < N || ldy < N)
-        return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row stride smaller than the row length");
-    if (K == 0) {
-        for (int64_t r = 0; r < B; ++r) std::fill(y + r * ldy, y + r * ldy + N, 0.0f);
+    return guarded(__func__, [&]() -> int {
+        scir::clear_error();
+        if (B < 0 || N < 0 || K < 0)
+            return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "negative size");
+        if (B == 0 || N == 0) return SCIR_HIP_OK;
+        if (!x || !y || (K > 0 && !b))
+            return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "null pointer argument");
+        if (ldx < N || ldy < N)
+            return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row stride smaller than the row length");
+        if (K == 0) {
+            for (int64_t r = 0; r < B; ++r) std::fill(y + r * ldy, y + r * ldy + N, 0.0f);
+            return SCIR_HIP_OK;
+        }
+        int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
+        nt = (int)std::min<int64_t>(nt, B);
+        auto work = [&](int w) {
+            for (int64_t r = w; r < B; r += nt) fir_row(x + r * ldx, b, K, N, y + r * ldy);
+        };
+        if (nt <= 1) {
+            work(0);
+            return SCIR_HIP_OK;
+        }
+        run_threads(nt, work);
         return SCIR_HIP_OK;
-    }
-    int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
-    nt = (int)std::min<int64_t>(nt, B);
-    auto work = [&](int w) {
-        for (int64_t r = w; r < B; r += nt) fir_row(x + r * ldx, b, K, N, y + r * ldy);
-    };
-    if (nt <= 1) {
-        work(0);
-        return SCIR_HIP_OK;
-    }
-    std::vector<std::thread> pool;
-    pool.reserve(nt);
-    for (int w = 0; w < nt; ++w) pool.emplace_back(work, w);
-    for (auto &t : pool) t.join();
-    return SCIR_HIP_OK;
+    });
 }
 
 // CPU arm of the polyphase resampler, in the reference's operation order
@@ -113,37 +132,50 @@ void resample_row(const double *xr, int64_t N, int64_t up, int64_t down, const d
 
 }  // namespace
 
-extern "C" int scir_hip_resample_len(int64_t N, int32_t up, int32_t down, int32_t K, int64_t *M);
+// Output length of resample_poly (crates/scir-signal/src/lib.rs:313-362):
+// every `down`-th sample of the full convolution from off = (K-1)/2 while
+// below len(conv) - off.
+extern "C" int scir_hip_resample_len(int64_t N, int32_t up, int32_t down, int32_t K, int64_t *M)
+{
+    scir::clear_error();
+    if (!M) return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "M is null");
+    if (N < 0 || up < 1 || down < 1 || K < 1)
+        return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "need N >= 0, up >= 1, down >= 1, K >= 1");
+    if (N > ((int64_t)1 << 40))
+        return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "N=%lld is too long", (long long)N);
+    const int64_t len = N * up + K - 1, off = (K - 1) / 2, end = len - off;
+    *M = end > off ? (end - off + down - 1) / down : 0;
+    return SCIR_HIP_OK;
+}
 
 extern "C" int scir_hip_resample_poly_f64_cpu(const double *x, int64_t ldx, int64_t B, int64_t N,
                                               int32_t up, int32_t down, const double *h,
                                               int32_t K, double *y, int64_t ldy,
                                               int32_t nthreads)
 {
-    int64_t M = 0;
-    const int st = scir_hip_resample_len(N, up, down, K, &M);
-    if (st != SCIR_HIP_OK) return st;
-    if (B < 0) return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "negative row count");
-    if (B == 0 || M == 0) return SCIR_HIP_OK;
-    if (!x || !y || !h)
-        return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "null pointer argument");
-    if (ldx < N || ldy < M)
-        return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row stride smaller than the row length");
-    int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
-    nt = (int)std::min<int64_t>(nt, B);
-    auto work = [&](int w) {
-        for (int64_t r = w; r < B; r += nt)
-            resample_row(x + r * ldx, N, up, down, h, K, y + r * ldy, M);
-    };
-    if (nt <= 1) {
-        work(0);
+    return guarded(__func__, [&]() -> int {
+        int64_t M = 0;
+        const int st = scir_hip_resample_len(N, up, down, K, &M);
+        if (st != SCIR_HIP_OK) return st;
+        if (B < 0) return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "negative row count");
+        if (B == 0 || M == 0) return SCIR_HIP_OK;
+        if (!x || !y || !h)
+            return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "null pointer argument");
+        if (ldx < N || ldy < M)
+            return scir::fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row stride smaller than the row length");
+        int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
+        nt = (int)std::min<int64_t>(nt, B);
+        auto work = [&](int w) {
+            for (int64_t r = w; r < B; r += nt)
+                resample_row(x + r * ldx, N, up, down, h, K, y + r * ldy, M);
+        };
+        if (nt <= 1) {
+            work(0);
+            return SCIR_HIP_OK;
+        }
+        run_threads(nt, work);
         return SCIR_HIP_OK;
-    }
-    std::vector<std::thread> pool;
-    pool.reserve(nt);
-    for (int w = 0; w < nt; ++w) pool.emplace_back(work, w);
-    for (auto &t : pool) t.join();
-    return SCIR_HIP_OK;
+    });
 }
 
 // CPU arm of sosfilt / filtfilt (crates/scir-signal/src/lib.rs:260-291): the
@@ -199,10 +231,7 @@ int sos_cpu(const double *x, int64_t ldx, int64_t rows, int64_t N, const double 
         work(0);
         return SCIR_HIP_OK;
     }
-    std::vector<std::thread> pool;
-    pool.reserve(nt);
-    for (int w = 0; w < nt; ++w) pool.emplace_back(work, w);
-    for (auto &t : pool) t.join();
+    run_threads(nt, work);
     return SCIR_HIP_OK;
 }
 
@@ -212,12 +241,16 @@ extern "C" int scir_hip_sosfilt_f64_cpu(const double *x, int64_t ldx, int64_t ro
                                         const double *sos, int32_t S, double *y, int64_t ldy,
                                         int32_t nthreads)
 {
-    return sos_cpu(x, ldx, rows, n, sos, S, y, ldy, nthreads, false);
+    return guarded(__func__, [&]() -> int {
+        return sos_cpu(x, ldx, rows, n, sos, S, y, ldy, nthreads, false);
+    });
 }
 
 extern "C" int scir_hip_filtfilt_f64_cpu(const double *x, int64_t ldx, int64_t rows, int64_t n,
                                          const double *sos, int32_t S, double *y, int64_t ldy,
                                          int32_t nthreads)
 {
-    return sos_cpu(x, ldx, rows, n, sos, S, y, ldy, nthreads, true);
+    return guarded(__func__, [&]() -> int {
+        return sos_cpu(x, ldx, rows, n, sos, S, y, ldy, nthreads, true);
+    });
 }

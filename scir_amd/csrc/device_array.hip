@@ -10,6 +10,8 @@
 // in the reference's CPU loops (lib.rs:206-260), so it is bit-identical.
 #include "common.hpp"
 
+#include <memory>
+
 namespace scir {
 namespace {
 
@@ -62,84 +64,92 @@ extern "C" {
 
 int scir_hip_buffer_create(size_t bytes, scir_hip_buffer *buf)
 {
-    clear_error();
-    if (!buf) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "buf is null");
-    *buf = nullptr;
-    DeviceState *st = nullptr;
-    SCIR_TRY(current_device_state(&st));
-    auto *b = new scir_hip_buffer_s;
-    b->device = st->device;
-    b->bytes = bytes;
-    if (bytes > 0) {
-        hipError_t e = hipMalloc(&b->ptr, bytes);
-        if (e != hipSuccess) {
-            delete b;
-            return fail(SCIR_HIP_ERR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", bytes,
-                        hipGetErrorString(e));
-        }
-    }
-    *buf = b;
-    return SCIR_HIP_OK;
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        if (!buf) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "buf is null");
+        *buf = nullptr;
+        DeviceState *st = nullptr;
+        SCIR_TRY(current_device_state(&st));
+        host_alloc_point();
+        std::unique_ptr<scir_hip_buffer_s> b(new scir_hip_buffer_s);
+        b->device = st->device;
+        b->bytes = bytes;
+        if (bytes > 0) SCIR_TRY(device_alloc(&b->ptr, bytes, "scir_hip_buffer_create"));
+        *buf = b.release();
+        return SCIR_HIP_OK;
+    });
 }
 
 int scir_hip_buffer_upload(scir_hip_buffer buf, const void *host, size_t bytes)
 {
-    clear_error();
-    if (!buf || (bytes > 0 && !host)) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "null argument");
-    if (bytes > buf->bytes)
-        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "upload of %zu bytes into %zu", bytes,
-                    buf->bytes);
-    if (bytes > 0) SCIR_HIP_TRY(hipMemcpy(buf->ptr, host, bytes, hipMemcpyHostToDevice));
-    return SCIR_HIP_OK;
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        if (!buf || (bytes > 0 && !host)) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "null argument");
+        if (bytes > buf->bytes)
+            return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "upload of %zu bytes into %zu", bytes,
+                        buf->bytes);
+        if (bytes > 0) SCIR_HIP_TRY(hipMemcpy(buf->ptr, host, bytes, hipMemcpyHostToDevice));
+        return SCIR_HIP_OK;
+    });
 }
 
 int scir_hip_buffer_download(scir_hip_buffer buf, void *host, size_t bytes)
 {
-    clear_error();
-    if (!buf || (bytes > 0 && !host)) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "null argument");
-    if (bytes > buf->bytes)
-        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "download of %zu bytes from %zu", bytes,
-                    buf->bytes);
-    // hipMemcpy waits for work on the null stream; callers that used other
-    // streams synchronise them first.
-    if (bytes > 0) SCIR_HIP_TRY(hipMemcpy(host, buf->ptr, bytes, hipMemcpyDeviceToHost));
-    return SCIR_HIP_OK;
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        if (!buf || (bytes > 0 && !host)) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "null argument");
+        if (bytes > buf->bytes)
+            return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "download of %zu bytes from %zu", bytes,
+                        buf->bytes);
+        // hipMemcpy waits for work on the null stream; callers that used other
+        // streams synchronise them first.
+        if (bytes > 0) SCIR_HIP_TRY(hipMemcpy(host, buf->ptr, bytes, hipMemcpyDeviceToHost));
+        return SCIR_HIP_OK;
+    });
 }
 
 void *scir_hip_buffer_device_ptr(scir_hip_buffer buf) { return buf ? buf->ptr : nullptr; }
 
 int scir_hip_buffer_destroy(scir_hip_buffer buf)
 {
-    clear_error();
-    if (!buf) return SCIR_HIP_OK;
-    if (buf->ptr) {
-        // No kernel may still use the memory: wait for the buffer's own
-        // device, then restore the caller's current device.
-        int cur = -1;
-        SCIR_HIP_TRY(hipGetDevice(&cur));
-        if (cur != buf->device) SCIR_HIP_TRY(hipSetDevice(buf->device));
-        hipError_t e = hipDeviceSynchronize();
-        if (e == hipSuccess) e = hipFree(buf->ptr);
-        if (cur != buf->device) (void)hipSetDevice(cur);
-        SCIR_HIP_TRY(e);
-    }
-    delete buf;
-    return SCIR_HIP_OK;
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        if (!buf) return SCIR_HIP_OK;
+        if (buf->ptr) {
+            // No kernel may still use the memory: wait for the buffer's own
+            // device, then restore the caller's current device.
+            int cur = -1;
+            SCIR_HIP_TRY(hipGetDevice(&cur));
+            if (cur != buf->device) SCIR_HIP_TRY(hipSetDevice(buf->device));
+            hipError_t e = hipDeviceSynchronize();
+            if (e == hipSuccess) e = hipFree(buf->ptr);
+            if (cur != buf->device) (void)hipSetDevice(cur);
+            SCIR_HIP_TRY(e);
+        }
+        delete buf;
+        return SCIR_HIP_OK;
+    });
 }
 
 int scir_hip_add_scalar_f32_dev(const float *a, float alpha, float *out, int64_t n, void *stream)
 {
-    return launch_elementwise<0>(a, nullptr, alpha, out, n, stream);
+    return guarded(__func__, [&]() -> int {
+        return launch_elementwise<0>(a, nullptr, alpha, out, n, stream);
+    });
 }
 
 int scir_hip_mul_scalar_f32_dev(const float *a, float alpha, float *out, int64_t n, void *stream)
 {
-    return launch_elementwise<1>(a, nullptr, alpha, out, n, stream);
+    return guarded(__func__, [&]() -> int {
+        return launch_elementwise<1>(a, nullptr, alpha, out, n, stream);
+    });
 }
 
 int scir_hip_add_f32_dev(const float *a, const float *b, float *out, int64_t n, void *stream)
 {
-    return launch_elementwise<2>(a, b, 0.0f, out, n, stream);
+    return guarded(__func__, [&]() -> int {
+        return launch_elementwise<2>(a, b, 0.0f, out, n, stream);
+    });
 }
 
 }  // extern "C"

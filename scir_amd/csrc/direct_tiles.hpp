@@ -20,13 +20,60 @@
 // EXACT = true keeps the reference's arithmetic: separately rounded multiply
 // and add, taps accumulated in the order k = 0..K-1 from +0.0f
 // (crates/scir-gpu/src/lib.rs:1141-1148 adds taps[K-1-t]*x[i-t] for t
-// ascending, and b[t] == taps[K-1-t]).  Extra terms before the row start add
-// +-0, which leaves every bit unchanged (the running sum is never -0.0).
+// ascending, and b[t] == taps[K-1-t]).  For outputs i < K-1 the tile loop
+// also forms b[k]*0 for k > i (the zeros staged before the row start), which
+// the reference loop never forms.  With a finite b[k] such a term is +-0 and
+// leaves every bit unchanged (the running sum is never -0.0); with b[k] = Inf
+// or NaN it makes the output NaN, so every non-finite output i < K-1 is
+// recomputed from the reference loop's terms alone (fix_ramp_in).
 #pragma once
 
 #include "common.hpp"
 
 namespace scir {
+
+// Output i (logical index) of a row, from the reference loop's terms only:
+// sum_{k=0}^{min(i, K-1)} b[k] * x[i-k] in ascending k from +0.0f, with a
+// separately rounded multiply and add (EXACT, the order of
+// crates/scir-gpu/src/lib.rs:1141-1148) or a fused multiply-add.  x is read
+// from global memory; with `rev` logical sample t is physical N-1-t.  This is
+// the slow path every kernel takes for an output that came out non-finite:
+// zero padding (the direct tiles' ramp-in, a Toeplitz block's zero taps, an
+// FFT segment) spreads an Inf or NaN to outputs whose reference terms are
+// all finite, and this recomputation gives the reference loop's value class.
+template <bool EXACT>
+__device__ float direct_output(const float *__restrict__ xrow, int64_t N, bool rev,
+                               const float *__restrict__ b, int K, int64_t i)
+{
+    float acc = 0.0f;
+    const int64_t kend = i < (int64_t)K - 1 ? i : (int64_t)K - 1;
+#pragma clang loop unroll(disable)
+    for (int64_t k = 0; k <= kend; ++k) {
+        const int64_t t = i - k;
+        const float xv = xrow[rev ? N - 1 - t : t];
+        if (EXACT) {
+#pragma clang fp contract(off)
+            const float p = b[k] * xv;
+            acc = acc + p;
+        } else {
+            acc = __builtin_fmaf(b[k], xv, acc);
+        }
+    }
+    return acc;
+}
+
+// direct_output for an output of the ramp-in (i < K-1, i < N); any other
+// output keeps the value v the tile loop computed.  Not inlined: the slow
+// path stays out of the register allocation of the kernels' tile loops.
+template <bool EXACT>
+__device__ __noinline__ float ramp_in_output(float v, const float *__restrict__ xrow, int64_t N,
+                                             bool rev, const float *__restrict__ b, int K,
+                                             int64_t i)
+{
+    if (i >= (int64_t)K - 1 || i >= N) return v;
+    return direct_output<EXACT>(xrow, N, rev, b, K, i);
+}
+
 namespace direct {
 
 constexpr int kThreads = 256;
@@ -160,6 +207,25 @@ __device__ __forceinline__ void taps_from_lds(float (&acc)[R], const float *xs, 
             taps_chunk<R, 1, 1, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc + 2);
     } else if (rem & 1) {  // window starts 3 below the aligned index
         taps_chunk<R, 1, 3, EXACT>(acc, xs, H + lane_base - kc - 4, bs + kc);
+    }
+}
+
+// A lane's outputs i0 .. i0+R-1 (logical): each non-finite one below K-1 --
+// the only outputs whose tile sums include staged zeros from before the row
+// start -- is recomputed from the reference loop's terms (direct_output).
+template <int R, bool EXACT>
+__device__ __forceinline__ void fix_ramp_in(float (&acc)[R], const float *__restrict__ xrow,
+                                            int64_t N, bool rev, const float *__restrict__ b,
+                                            int K, int64_t i0)
+{
+    if (i0 >= (int64_t)K - 1) return;
+    unsigned bad = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) bad |= __builtin_isfinite(acc[r]) ? 0u : 1u << r;
+    if (bad) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (bad & (1u << r)) acc[r] = ramp_in_output<EXACT>(acc[r], xrow, N, rev, b, K, i0 + r);
     }
 }
 

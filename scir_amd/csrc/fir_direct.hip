@@ -13,8 +13,10 @@
 //     the LDS copy, so each staged sample is read from LDS ~(R+32)/R times
 //     and each tap feeds R multiply-adds from one scalar register;
 //   * taps are read with uniform (scalar) loads and applied 32 at a time;
-//     remainders use chunks of 16, 8, 4, 2, 1 taps, so no tap is padded and
-//     non-finite inputs behave as in the reference loop;
+//     remainders use chunks of 16, 8, 4, 2, 1 taps, so no tap is padded;
+//   * a non-finite output among the first K-1 of a row is recomputed from
+//     the reference loop's terms alone (fix_ramp_in), so Inf and NaN in x or
+//     b give non-finite outputs exactly where the reference loop does;
 //   * long filters are processed in super-chunks of 2048 taps so the LDS
 //     footprint stays bounded for any K.
 //
@@ -27,9 +29,11 @@
 //                  the order k = 0..K-1 from +0.0f, which is the reference
 //                  CPU loop's order (crates/scir-gpu/src/lib.rs:1141-1148:
 //                  taps[K-1-t]*x[i-t] for t ascending == b[t]*x[i-t]).  Extra
-//                  terms past the row start add +-0 to the sum, which leaves
-//                  every bit unchanged (the sum never becomes -0.0), so the
-//                  output is bit-identical to the reference.
+//                  terms past the row start add +-0 to the sum for finite
+//                  taps, which leaves every bit unchanged (the sum never
+//                  becomes -0.0); outputs they make non-finite are
+//                  recomputed without them.  The output is bit-identical to
+//                  the reference for any input, Inf and NaN included.
 #include "direct_tiles.hpp"
 
 #include <algorithm>
@@ -65,6 +69,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_kernel(
         __syncthreads();
         taps_from_lds<R, EXACT>(acc, xs, H, lane_base, kcount, b + ks);
     }
+    fix_ramp_in<R, EXACT>(acc, xrow, N, rev != 0, b, K, t0 + lane_base);
     store_outputs<R>(acc, y + row * ldy, t0 + lane_base, N, vec_y != 0, rev != 0);
 }
 
@@ -115,6 +120,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_pipe_kernel(
         if (it + 1 < ntiles) fetch(tfirst + it + 1);
         float acc[R];
         taps_from_lds<R, EXACT, true>(acc, buf, H, lane_base, K, b);
+        fix_ramp_in<R, EXACT>(acc, xrow, N, rev != 0, b, K, (tfirst + it) * T + lane_base);
         store_outputs<R>(acc, yrow, (tfirst + it) * T + lane_base, N, vec_y != 0, rev != 0);
     }
 }
@@ -225,6 +231,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_dma_kernel(
         float acc[R];
         taps_from_lds<R, EXACT, true>(acc, buf, H, lane_base, K, b);
         const int64_t i0 = (tfirst + it) * T + lane_base;
+        fix_ramp_in<R, EXACT>(acc, x + row * ldx, N, false, b, K, i0);
 #pragma unroll
         for (int r = 0; r < R; r += 4)
             store_dwordx4(yr, (unsigned)(4 * (i0 + r)),
@@ -397,57 +404,61 @@ extern "C" {
 int scir_hip_fir_f32_dev(const float *x, int64_t ldx, const float *b, int32_t K, int64_t B,
                          int64_t N, float *y, int64_t ldy, int32_t precision, void *stream)
 {
-    clear_error();
-    SCIR_TRY(check_dense_args(x, ldx, b, K, B, N, y, ldy));
-    if (B == 0 || N == 0) return SCIR_HIP_OK;
-    DeviceState *st = nullptr;
-    SCIR_TRY(current_device_state(&st));
-    std::lock_guard<std::mutex> g(st->mu);
-    return fir_dense_dispatch(x, ldx, b, K, B, N, y, ldy, precision, false, as_stream(stream),
-                              st);
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        SCIR_TRY(check_dense_args(x, ldx, b, K, B, N, y, ldy));
+        if (B == 0 || N == 0) return SCIR_HIP_OK;
+        DeviceState *st = nullptr;
+        SCIR_TRY(current_device_state(&st));
+        std::lock_guard<std::mutex> g(st->mu);
+        return fir_dense_dispatch(x, ldx, b, K, B, N, y, ldy, precision, false, as_stream(stream),
+                                  st);
+    });
 }
 
 int scir_hip_fir_f32_host(const float *x, int64_t ldx, const float *b, int32_t K, int64_t B,
                           int64_t N, float *y, int64_t ldy, int32_t precision)
 {
-    clear_error();
-    SCIR_TRY(check_dense_args(x, ldx, b, K, B, N, y, ldy));
-    if (B == 0 || N == 0) return SCIR_HIP_OK;
-    if (precision < SCIR_HIP_PREC_F32 || precision > SCIR_HIP_PREC_FFT32)
-        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "unknown precision %d", precision);
-    DeviceState *st = nullptr;
-    SCIR_TRY(current_device_state(&st));
-    std::lock_guard<std::mutex> g(st->mu);
-    if (!st->host_stream) SCIR_HIP_TRY(hipStreamCreateWithFlags(&st->host_stream, hipStreamNonBlocking));
-    hipStream_t s = st->host_stream;
-    const size_t row_bytes = (size_t)N * sizeof(float);
-    // Device rows are padded to a multiple of 4 floats so the kernels can use
-    // 16-byte accesses.
-    const int64_t ldd = (N + 3) & ~int64_t(3);
-    const size_t mat_bytes = (size_t)B * (size_t)ldd * sizeof(float);
-    SCIR_TRY(ensure_buffer(st->host_x, mat_bytes, s));
-    SCIR_TRY(ensure_buffer(st->host_y, mat_bytes, s));
-    SCIR_TRY(ensure_buffer(st->host_b, (size_t)std::max(K, 1) * sizeof(float), s));
-    float *dx = static_cast<float *>(st->host_x.ptr);
-    float *dy = static_cast<float *>(st->host_y.ptr);
-    float *db = static_cast<float *>(st->host_b.ptr);
-    // One linear copy when host and device rows are both packed (a pitched
-    // copy of many rows runs as many small transfers).
-    if (ldx == N && ldd == N)
-        SCIR_HIP_TRY(hipMemcpyAsync(dx, x, mat_bytes, hipMemcpyHostToDevice, s));
-    else
-        SCIR_HIP_TRY(hipMemcpy2DAsync(dx, ldd * sizeof(float), x, ldx * sizeof(float), row_bytes,
-                                      B, hipMemcpyHostToDevice, s));
-    if (K > 0)
-        SCIR_HIP_TRY(hipMemcpyAsync(db, b, (size_t)K * sizeof(float), hipMemcpyHostToDevice, s));
-    SCIR_TRY(fir_dense_dispatch(dx, ldd, db, K, B, N, dy, ldd, precision, false, s, st));
-    if (ldy == N && ldd == N)
-        SCIR_HIP_TRY(hipMemcpyAsync(y, dy, mat_bytes, hipMemcpyDeviceToHost, s));
-    else
-        SCIR_HIP_TRY(hipMemcpy2DAsync(y, ldy * sizeof(float), dy, ldd * sizeof(float), row_bytes,
-                                      B, hipMemcpyDeviceToHost, s));
-    SCIR_HIP_TRY(hipStreamSynchronize(s));
-    return SCIR_HIP_OK;
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        SCIR_TRY(check_dense_args(x, ldx, b, K, B, N, y, ldy));
+        if (B == 0 || N == 0) return SCIR_HIP_OK;
+        if (precision < SCIR_HIP_PREC_F32 || precision > SCIR_HIP_PREC_FFT32)
+            return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "unknown precision %d", precision);
+        DeviceState *st = nullptr;
+        SCIR_TRY(current_device_state(&st));
+        std::lock_guard<std::mutex> g(st->mu);
+        if (!st->host_stream) SCIR_HIP_TRY(hipStreamCreateWithFlags(&st->host_stream, hipStreamNonBlocking));
+        hipStream_t s = st->host_stream;
+        const size_t row_bytes = (size_t)N * sizeof(float);
+        // Device rows are padded to a multiple of 4 floats so the kernels can use
+        // 16-byte accesses.
+        const int64_t ldd = (N + 3) & ~int64_t(3);
+        const size_t mat_bytes = (size_t)B * (size_t)ldd * sizeof(float);
+        SCIR_TRY(ensure_buffer(st->host_x, mat_bytes, s));
+        SCIR_TRY(ensure_buffer(st->host_y, mat_bytes, s));
+        SCIR_TRY(ensure_buffer(st->host_b, (size_t)std::max(K, 1) * sizeof(float), s));
+        float *dx = static_cast<float *>(st->host_x.ptr);
+        float *dy = static_cast<float *>(st->host_y.ptr);
+        float *db = static_cast<float *>(st->host_b.ptr);
+        // One linear copy when host and device rows are both packed (a pitched
+        // copy of many rows runs as many small transfers).
+        if (ldx == N && ldd == N)
+            SCIR_HIP_TRY(hipMemcpyAsync(dx, x, mat_bytes, hipMemcpyHostToDevice, s));
+        else
+            SCIR_HIP_TRY(hipMemcpy2DAsync(dx, ldd * sizeof(float), x, ldx * sizeof(float), row_bytes,
+                                          B, hipMemcpyHostToDevice, s));
+        if (K > 0)
+            SCIR_HIP_TRY(hipMemcpyAsync(db, b, (size_t)K * sizeof(float), hipMemcpyHostToDevice, s));
+        SCIR_TRY(fir_dense_dispatch(dx, ldd, db, K, B, N, dy, ldd, precision, false, s, st));
+        if (ldy == N && ldd == N)
+            SCIR_HIP_TRY(hipMemcpyAsync(y, dy, mat_bytes, hipMemcpyDeviceToHost, s));
+        else
+            SCIR_HIP_TRY(hipMemcpy2DAsync(y, ldy * sizeof(float), dy, ldd * sizeof(float), row_bytes,
+                                          B, hipMemcpyDeviceToHost, s));
+        SCIR_HIP_TRY(hipStreamSynchronize(s));
+        return SCIR_HIP_OK;
+    });
 }
 
 }  // extern "C"

@@ -23,7 +23,14 @@
 // Error: the FFT's rounding is normwise, about log2(NF) * 2^-24 * |x|_2 |b|_1
 // per segment, not relative to each output; the direct and MFMA kernels keep
 // the elementwise (K+1) 2^-24 sum|b||x| bound.
-#include "common.hpp"
+//
+// Non-finite values: an Inf or NaN anywhere in a segment's window (or an
+// overflow inside the transform) makes every output of the segment NaN.  The
+// epilogue recomputes each non-finite output from the reference loop's terms
+// (direct_output with the whole filter), so outputs are non-finite exactly
+// where the reference loop's are.  With a partitioned filter the check is
+// made by the block that writes the output last.
+#include "direct_tiles.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -265,12 +272,16 @@ __global__ __launch_bounds__(Plan<LOGN>::T) void fir_fft_spectrum_kernel(
 // V = NF - K + 1, o = (s % spr) * V.  REV reads and writes every row back to
 // front (the anti-causal correlation of the dL/dx adjoint).  With `accumulate`
 // the outputs are added to y instead of stored (the later tap blocks of a
-// partitioned filter).
+// partitioned filter).  The block holds taps bfull[j0, j0 + K) of a filter of
+// Kfull taps: its output t is output t + j0 of the whole filter, its input t
+// is the row's input t, and it writes outputs t < final_len last (the later
+// blocks start at t = kFftBlockTaps).
 template <int LOGN, bool REV>
 __global__ __launch_bounds__(Plan<LOGN>::T) void fir_fft_kernel(
     const float *__restrict__ x, int64_t ldx, float *__restrict__ y, int64_t ldy, int64_t N,
     int K, const float2 *__restrict__ H, const float2 *__restrict__ tq_g, int64_t spr,
-    int64_t nseg, int accumulate)
+    int64_t nseg, int accumulate, const float *__restrict__ bfull, int Kfull, int64_t j0,
+    int64_t final_len)
 {
     constexpr int NF = 1 << LOGN, T = Plan<LOGN>::T;
     extern __shared__ float2 lds_f2[];
@@ -310,6 +321,15 @@ __global__ __launch_bounds__(Plan<LOGN>::T) void fir_fft_kernel(
     fft<LOGN, false, false>(d, tq, nullptr, tid);
     fft<LOGN, true, true>(d, tq, H, tid);
     float *ya = y + ra * ldy, *yb = y + rb * ldy;
+    // The rows of the whole filter (for recomputing non-finite outputs): in
+    // REV mode the block's rows start j0 samples into the physical row.
+    const float *fa = REV ? xa - j0 : xa, *fb = REV ? xb - j0 : xb;
+    const int64_t Nf = N + j0;
+    auto finish = [&](float v, const float *frow, int64_t t) {
+        if (!__builtin_isfinite(v) && t < final_len)
+            v = direct_output<false>(frow, Nf, REV, bfull, Kfull, t + j0);
+        return v;
+    };
     if (has1 && oa + V <= N && ob + V <= N) {
         float *qa = REV ? ya + (N - 1 - oa) : ya + oa;
         float *qb = REV ? yb + (N - 1 - ob) : yb + ob;
@@ -317,15 +337,16 @@ __global__ __launch_bounds__(Plan<LOGN>::T) void fir_fft_kernel(
 #pragma unroll 4
             for (int v = tid; v < V; v += T) {
                 const float2 o = d[pad(K - 1 + v)];
-                qa[REV ? -v : v] += o.x;
-                qb[REV ? -v : v] += o.y;
+                float &pa = qa[REV ? -v : v], &pb = qb[REV ? -v : v];
+                pa = finish(pa + o.x, fa, oa + v);
+                pb = finish(pb + o.y, fb, ob + v);
             }
         } else {
 #pragma unroll 4
             for (int v = tid; v < V; v += T) {
                 const float2 o = d[pad(K - 1 + v)];
-                qa[REV ? -v : v] = o.x;
-                qb[REV ? -v : v] = o.y;
+                qa[REV ? -v : v] = finish(o.x, fa, oa + v);
+                qb[REV ? -v : v] = finish(o.y, fb, ob + v);
             }
         }
     } else {
@@ -334,27 +355,22 @@ __global__ __launch_bounds__(Plan<LOGN>::T) void fir_fft_kernel(
             const int64_t ta = oa + v, tb = ob + v;
             if (ta < N) {
                 float &q = ya[REV ? N - 1 - ta : ta];
-                q = accumulate ? q + o.x : o.x;
+                q = finish(accumulate ? q + o.x : o.x, fa, ta);
             }
             if (has1 && tb < N) {
                 float &q = yb[REV ? N - 1 - tb : tb];
-                q = accumulate ? q + o.y : o.y;
+                q = finish(accumulate ? q + o.y : o.y, fb, tb);
             }
         }
     }
 }
 
-// Quarter-wave tables, one per size, built on first use per device.
-struct TwTables {
-    DeviceBuffer tab[3];
-};
-TwTables g_tw[64];
-
+// Twiddle tables, one per size, built on first use per device (kept in the
+// device state, freed by scir_hip_shutdown).
 int twiddle_table(DeviceState *st, int logn, hipStream_t s, const float2 **out)
 {
     const int NF = 1 << logn;
-    TwTables &t = g_tw[st->device & 63];
-    DeviceBuffer &buf = t.tab[logn - 12];
+    DeviceBuffer &buf = st->fft_twiddles[logn - 12];
     if (!buf.ptr) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         if (s != nullptr && hipStreamIsCapturing(s, &cs) == hipSuccess &&
@@ -375,9 +391,17 @@ int twiddle_table(DeviceState *st, int logn, hipStream_t s, const float2 **out)
             const double a = 2.0 * kPi * (double)m / (double)r01;
             h[kFine + NF / kFine + m] = make_float2((float)std::cos(a), (float)std::sin(a));
         }
-        SCIR_HIP_TRY(hipMalloc(&buf.ptr, h.size() * sizeof(float2)));
+        void *p = nullptr;
+        SCIR_TRY(device_alloc(&p, h.size() * sizeof(float2), "FFT twiddle table"));
+        const hipError_t e = hipMemcpy(p, h.data(), h.size() * sizeof(float2),
+                                       hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(p);
+            return fail(SCIR_HIP_ERR_BACKEND, "FFT twiddle upload failed: %s",
+                        hipGetErrorString(e));
+        }
+        buf.ptr = p;
         buf.bytes = h.size() * sizeof(float2);
-        SCIR_HIP_TRY(hipMemcpy(buf.ptr, h.data(), buf.bytes, hipMemcpyHostToDevice));
     }
     *out = static_cast<const float2 *>(buf.ptr);
     return SCIR_HIP_OK;
@@ -385,13 +409,14 @@ int twiddle_table(DeviceState *st, int logn, hipStream_t s, const float2 **out)
 
 template <int LOGN>
 int launch_fft(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N, float *y,
-               int64_t ldy, bool rev, bool accumulate, hipStream_t s, DeviceState *st)
+               int64_t ldy, bool rev, bool accumulate, const float *bfull, int Kfull, int64_t j0,
+               int64_t final_len, hipStream_t s, DeviceState *st)
 {
     constexpr int NF = 1 << LOGN, T = Plan<LOGN>::T;
     const float2 *tq = nullptr;
     SCIR_TRY(twiddle_table(st, LOGN, s, &tq));
     StreamScratch &sc = scratch_for(st, s);
-    SCIR_TRY(ensure_buffer(sc.fft_spectrum, (size_t)NF * sizeof(float2), s));
+    SCIR_TRY(ensure_scratch(st, sc.fft_spectrum, (size_t)NF * sizeof(float2), s));
     float2 *H = static_cast<float2 *>(sc.fft_spectrum.ptr);
     constexpr size_t lds = lds_bytes<LOGN>();
     SCIR_TRY(allow_dynamic_lds(reinterpret_cast<const void *>(&fir_fft_kernel<LOGN, false>),
@@ -410,10 +435,11 @@ int launch_fft(const float *x, int64_t ldx, const float *b, int K, int64_t B, in
     const int acc = accumulate ? 1 : 0;
     if (rev)
         hipLaunchKernelGGL((fir_fft_kernel<LOGN, true>), dim3((unsigned)blocks), dim3(T), lds, s, x,
-                           ldx, y, ldy, N, K, H, tq, spr, nseg, acc);
+                           ldx, y, ldy, N, K, H, tq, spr, nseg, acc, bfull, Kfull, j0, final_len);
     else
         hipLaunchKernelGGL((fir_fft_kernel<LOGN, false>), dim3((unsigned)blocks), dim3(T), lds, s,
-                           x, ldx, y, ldy, N, K, H, tq, spr, nseg, acc);
+                           x, ldx, y, ldy, N, K, H, tq, spr, nseg, acc, bfull, Kfull, j0,
+                           final_len);
     return check_launch("fir_fft_kernel");
 }
 
@@ -434,13 +460,18 @@ int fft_log_size(int K)
 }
 
 int fft_block(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N, float *y,
-              int64_t ldy, bool rev, bool accumulate, hipStream_t stream, DeviceState *st)
+              int64_t ldy, bool rev, bool accumulate, const float *bfull, int Kfull, int64_t j0,
+              int64_t final_len, hipStream_t stream, DeviceState *st)
 {
+#define SCIR_FFT_BLOCK(L)                                                                      \
+    launch_fft<L>(x, ldx, b, K, B, N, y, ldy, rev, accumulate, bfull, Kfull, j0, final_len, \
+                  stream, st)
     switch (fft_log_size(K)) {
-    case 12: return launch_fft<12>(x, ldx, b, K, B, N, y, ldy, rev, accumulate, stream, st);
-    case 13: return launch_fft<13>(x, ldx, b, K, B, N, y, ldy, rev, accumulate, stream, st);
-    default: return launch_fft<14>(x, ldx, b, K, B, N, y, ldy, rev, accumulate, stream, st);
+    case 12: return SCIR_FFT_BLOCK(12);
+    case 13: return SCIR_FFT_BLOCK(13);
+    default: return SCIR_FFT_BLOCK(14);
     }
+#undef SCIR_FFT_BLOCK
 }
 
 }  // namespace
@@ -460,7 +491,12 @@ int fir_fft_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, i
         // to front the same sums read x from sample j0 on and write y from 0.
         const float *xs = rev ? x + j0 : x;
         float *ys = rev ? y : y + j0;
-        SCIR_TRY(fft_block(xs, ldx, b + j0, Kj, B, N - j0, ys, ldy, rev, j0 > 0, stream, st));
+        // Block j writes outputs j0 .. j0 + kFftBlockTaps - 1 last (all the
+        // rest when it is the final block).
+        const bool last = j0 + kFftBlockTaps >= K || j0 + kFftBlockTaps >= N;
+        const int64_t final_len = last ? N - j0 : (int64_t)kFftBlockTaps;
+        SCIR_TRY(fft_block(xs, ldx, b + j0, Kj, B, N - j0, ys, ldy, rev, j0 > 0, b, K, j0,
+                           final_len, stream, st));
     }
     return SCIR_HIP_OK;
 }

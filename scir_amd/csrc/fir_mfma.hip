@@ -35,13 +35,18 @@
 //     products are kept: hh (own accumulator), hm, mh, mm, hl, lh (a second
 //     accumulator); the dropped ml, lm, ll are below 2^-24 relative.
 //   S = 1 (FP16): x and b rounded to fp16, one product.
-// Both assume finite inputs: a zero tap of the Toeplitz block still
-// multiplies its sample, so an Inf makes NaN in the outputs of its block.
+// Non-finite values: a zero tap of the Toeplitz block still multiplies its
+// sample (and a zero sample its tap), so an Inf or NaN spreads to every output
+// of the 32-output blocks it enters, and a finite value beyond the bf16/fp16
+// range splits into Inf.  The epilogue therefore recomputes every non-finite
+// output from the reference loop's terms (direct_output, FP32 fused
+// multiply-add): outputs are non-finite exactly where the reference loop's
+// are, at the cost of one class check per output when none is.
 //
 // MFMA 32x32x16 operand layout (lane l = 32*g + i, g = l/32, i = l%32):
 //   A: lane holds A[i][8g + e], e = 0..7;  B: lane holds B[8g + e][i];
 //   D: lane holds D[8q + 4g + j][i] in acc[4q + j], q, j = 0..3.
-#include "common.hpp"
+#include "direct_tiles.hpp"
 
 #include <algorithm>
 
@@ -388,11 +393,14 @@ __device__ __forceinline__ void contract_segment(Contraction<S, TW> &ct, const u
 }
 
 // Epilogue: tile t, lane (i, g) holds block P0 + wave*32*TW + t + TW*i,
-// samples 8q + 4g + j; outputs past N are dropped.
+// samples 8q + 4g + j; outputs past N are dropped.  A non-finite output is
+// recomputed from the reference loop's terms (x row xrow, taps b[0, K)).
 template <int S, int TW>
 __device__ __forceinline__ void store_segment(const Contraction<S, TW> &ct,
                                               float *__restrict__ yrow, int64_t N, int64_t P0,
-                                              int wave, int i, int g, int vec_y, int rev)
+                                              int wave, int i, int g, int vec_y, int rev,
+                                              const float *__restrict__ xrow,
+                                              const float *__restrict__ b, int K)
 {
 #pragma unroll
     for (int t = 0; t < TW; ++t) {
@@ -405,6 +413,13 @@ __device__ __forceinline__ void store_segment(const Contraction<S, TW> &ct,
             v.z = ct.acc0[t][4 * q + 2] + ct.acc1[t][4 * q + 2];
             v.w = ct.acc0[t][4 * q + 3] + ct.acc1[t][4 * q + 3];
             const int64_t n = n0 + 8 * q;
+            if (!__builtin_isfinite(v.x) || !__builtin_isfinite(v.y) ||
+                !__builtin_isfinite(v.z) || !__builtin_isfinite(v.w)) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (n + j < N && !__builtin_isfinite(v[j]))
+                        v[j] = direct_output<false>(xrow, N, rev != 0, b, K, n + j);
+            }
             if (vec_y && n + 3 < N) {
                 if (!rev)
                     *reinterpret_cast<f32x4 *>(yrow + n) = v;
@@ -486,7 +501,7 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
         Contraction<S, TW> ct;
         contract_segment<S, TW>(ct, lds, split_stride, tab_offset, copy_bytes, msteps, m_hi, wave,
                                 i, g);
-        store_segment<S, TW>(ct, y + row * ldy, N, P0, wave, i, g, vec_y, rev);
+        store_segment<S, TW>(ct, y + row * ldy, N, P0, wave, i, g, vec_y, rev, xrow, b, K);
     }
 }
 
@@ -551,7 +566,8 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_ragged_kernel(
             Contraction<S, TW> ct;
             contract_segment<S, TW>(ct, lds, split_stride, tab_offset, copy_bytes, msteps, m_hi,
                                     wave, i, g);
-            store_segment<S, TW>(ct, yrow, w.len, P0, wave, i, g, vec_y, 0);
+            store_segment<S, TW>(ct, yrow, w.len, P0, wave, i, g, vec_y, 0, xrow, b + w.b0,
+                                 w.K);
         }
     }
 }

@@ -114,21 +114,6 @@ __global__ __launch_bounds__(kThreads) void resample_poly_kernel(
 
 using namespace scir;
 
-extern "C" {
-
-int scir_hip_resample_len(int64_t N, int32_t up, int32_t down, int32_t K, int64_t *M)
-{
-    clear_error();
-    if (!M) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "M is null");
-    if (N < 0 || up < 1 || down < 1 || K < 1)
-        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "need N >= 0, up >= 1, down >= 1, K >= 1");
-    const int64_t len = N * up + K - 1, off = (K - 1) / 2, end = len - off;
-    *M = end > off ? (end - off + down - 1) / down : 0;
-    return SCIR_HIP_OK;
-}
-
-}  // extern "C"
-
 namespace {
 
 // Launch on `s`; the caller holds st->mu.
@@ -187,48 +172,52 @@ int scir_hip_resample_poly_f64_dev(const double *x, int64_t ldx, int64_t B, int6
                                    int32_t down, const double *h, int32_t K, double *y,
                                    int64_t ldy, void *stream)
 {
-    int64_t M = 0;
-    SCIR_TRY(resample_args(x, ldx, B, N, up, down, h, K, y, ldy, &M));
-    if (B == 0 || M == 0) return SCIR_HIP_OK;
-    DeviceState *st = nullptr;
-    SCIR_TRY(current_device_state(&st));
-    std::lock_guard<std::mutex> lk(st->mu);
-    return resample_launch(x, ldx, B, N, up, down, h, K, y, ldy, M, as_stream(stream));
+    return guarded(__func__, [&]() -> int {
+        int64_t M = 0;
+        SCIR_TRY(resample_args(x, ldx, B, N, up, down, h, K, y, ldy, &M));
+        if (B == 0 || M == 0) return SCIR_HIP_OK;
+        DeviceState *st = nullptr;
+        SCIR_TRY(current_device_state(&st));
+        std::lock_guard<std::mutex> lk(st->mu);
+        return resample_launch(x, ldx, B, N, up, down, h, K, y, ldy, M, as_stream(stream));
+    });
 }
 
 int scir_hip_resample_poly_f64_host(const double *x, int64_t ldx, int64_t B, int64_t N,
                                     int32_t up, int32_t down, const double *h, int32_t K,
                                     double *y, int64_t ldy)
 {
-    int64_t M = 0;
-    SCIR_TRY(resample_args(x, ldx, B, N, up, down, h, K, y, ldy, &M));
-    if (B == 0 || M == 0) return SCIR_HIP_OK;
-    DeviceState *st = nullptr;
-    SCIR_TRY(current_device_state(&st));
-    std::lock_guard<std::mutex> lk(st->mu);
-    if (!st->host_stream)
-        SCIR_HIP_TRY(hipStreamCreateWithFlags(&st->host_stream, hipStreamNonBlocking));
-    hipStream_t s = st->host_stream;
-    SCIR_TRY(ensure_buffer(st->host_x, (size_t)B * N * sizeof(double), s));
-    SCIR_TRY(ensure_buffer(st->host_y, (size_t)B * M * sizeof(double), s));
-    SCIR_TRY(ensure_buffer(st->host_b, (size_t)K * sizeof(double), s));
-    double *dx = static_cast<double *>(st->host_x.ptr);
-    double *dy = static_cast<double *>(st->host_y.ptr);
-    double *dh = static_cast<double *>(st->host_b.ptr);
-    if (ldx == N)
-        SCIR_HIP_TRY(hipMemcpyAsync(dx, x, (size_t)B * N * sizeof(double), hipMemcpyHostToDevice, s));
-    else
-        SCIR_HIP_TRY(hipMemcpy2DAsync(dx, N * sizeof(double), x, ldx * sizeof(double),
-                                      N * sizeof(double), B, hipMemcpyHostToDevice, s));
-    SCIR_HIP_TRY(hipMemcpyAsync(dh, h, (size_t)K * sizeof(double), hipMemcpyHostToDevice, s));
-    SCIR_TRY(resample_launch(dx, N, B, N, up, down, dh, K, dy, M, M, s));
-    if (ldy == M)
-        SCIR_HIP_TRY(hipMemcpyAsync(y, dy, (size_t)B * M * sizeof(double), hipMemcpyDeviceToHost, s));
-    else
-        SCIR_HIP_TRY(hipMemcpy2DAsync(y, ldy * sizeof(double), dy, M * sizeof(double),
-                                      M * sizeof(double), B, hipMemcpyDeviceToHost, s));
-    SCIR_HIP_TRY(hipStreamSynchronize(s));
-    return SCIR_HIP_OK;
+    return guarded(__func__, [&]() -> int {
+        int64_t M = 0;
+        SCIR_TRY(resample_args(x, ldx, B, N, up, down, h, K, y, ldy, &M));
+        if (B == 0 || M == 0) return SCIR_HIP_OK;
+        DeviceState *st = nullptr;
+        SCIR_TRY(current_device_state(&st));
+        std::lock_guard<std::mutex> lk(st->mu);
+        if (!st->host_stream)
+            SCIR_HIP_TRY(hipStreamCreateWithFlags(&st->host_stream, hipStreamNonBlocking));
+        hipStream_t s = st->host_stream;
+        SCIR_TRY(ensure_buffer(st->host_x, (size_t)B * N * sizeof(double), s));
+        SCIR_TRY(ensure_buffer(st->host_y, (size_t)B * M * sizeof(double), s));
+        SCIR_TRY(ensure_buffer(st->host_b, (size_t)K * sizeof(double), s));
+        double *dx = static_cast<double *>(st->host_x.ptr);
+        double *dy = static_cast<double *>(st->host_y.ptr);
+        double *dh = static_cast<double *>(st->host_b.ptr);
+        if (ldx == N)
+            SCIR_HIP_TRY(hipMemcpyAsync(dx, x, (size_t)B * N * sizeof(double), hipMemcpyHostToDevice, s));
+        else
+            SCIR_HIP_TRY(hipMemcpy2DAsync(dx, N * sizeof(double), x, ldx * sizeof(double),
+                                          N * sizeof(double), B, hipMemcpyHostToDevice, s));
+        SCIR_HIP_TRY(hipMemcpyAsync(dh, h, (size_t)K * sizeof(double), hipMemcpyHostToDevice, s));
+        SCIR_TRY(resample_launch(dx, N, B, N, up, down, dh, K, dy, M, M, s));
+        if (ldy == M)
+            SCIR_HIP_TRY(hipMemcpyAsync(y, dy, (size_t)B * M * sizeof(double), hipMemcpyDeviceToHost, s));
+        else
+            SCIR_HIP_TRY(hipMemcpy2DAsync(y, ldy * sizeof(double), dy, M * sizeof(double),
+                                          M * sizeof(double), B, hipMemcpyDeviceToHost, s));
+        SCIR_HIP_TRY(hipStreamSynchronize(s));
+        return SCIR_HIP_OK;
+    });
 }
 
 }  // extern "C"

@@ -314,7 +314,7 @@ int vjp_db_run(const float *g, int64_t ldg, const float *x, int64_t ldx, int32_t
     const TuneChoice tc = tune_lookup(K, N);
     if (tc.vjp_db_mfma) {
         StreamScratch &sc = scratch_for(st, s);
-        SCIR_TRY(ensure_buffer(sc.partials, (size_t)nwg * K * sizeof(float), s));
+        SCIR_TRY(ensure_scratch(st, sc.partials, (size_t)nwg * K * sizeof(float), s));
         float *partial = static_cast<float *>(sc.partials.ptr);
         hipLaunchKernelGGL(fir_vjp_db_mfma_kernel, dim3((unsigned)nwg), dim3(kThreads), 0, s, g,
                            ldg, x, ldx, (int)K, N, (int)segs, (int)tiles_per_wg, partial,
@@ -334,7 +334,7 @@ int vjp_db_run(const float *g, int64_t ldg, const float *x, int64_t ldx, int32_t
     SCIR_TRY(allow_dynamic_lds(reinterpret_cast<const void *>(&fir_vjp_db_partial_kernel),
                                160 * 1024));
     StreamScratch &sc = scratch_for(st, s);
-    SCIR_TRY(ensure_buffer(sc.partials, (size_t)nwg * K * sizeof(float), s));
+    SCIR_TRY(ensure_scratch(st, sc.partials, (size_t)nwg * K * sizeof(float), s));
     float *partial = static_cast<float *>(sc.partials.ptr);
     hipLaunchKernelGGL(fir_vjp_db_partial_kernel, dim3((unsigned)nwg), dim3(kThreads), lds, s, g,
                        ldg, x, ldx, (int)K, N, (int)segs, (int)tiles_per_wg, hist, partial,
@@ -354,73 +354,79 @@ extern "C" {
 int scir_hip_fir_vjp_dx_dev(const float *g, int64_t ldg, const float *b, int32_t K, int64_t B,
                             int64_t N, float *dx, int64_t lddx, int32_t precision, void *stream)
 {
-    clear_error();
-    SCIR_TRY(check_dense_args(g, ldg, b, K, B, N, dx, lddx));
-    if (B == 0 || N == 0) return SCIR_HIP_OK;
-    DeviceState *st = nullptr;
-    SCIR_TRY(current_device_state(&st));
-    std::lock_guard<std::mutex> lk(st->mu);
-    return fir_dense_dispatch(g, ldg, b, K, B, N, dx, lddx, precision, true, as_stream(stream), st);
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        SCIR_TRY(check_dense_args(g, ldg, b, K, B, N, dx, lddx));
+        if (B == 0 || N == 0) return SCIR_HIP_OK;
+        DeviceState *st = nullptr;
+        SCIR_TRY(current_device_state(&st));
+        std::lock_guard<std::mutex> lk(st->mu);
+        return fir_dense_dispatch(g, ldg, b, K, B, N, dx, lddx, precision, true, as_stream(stream), st);
+    });
 }
 
 int scir_hip_fir_vjp_db_dev(const float *g, int64_t ldg, const float *x, int64_t ldx, int32_t K,
                             int64_t B, int64_t N, float *db, void *stream)
 {
-    clear_error();
-    SCIR_TRY(vjp_db_args(g, ldg, x, ldx, K, B, N, db));
-    if (K == 0) return SCIR_HIP_OK;
-    DeviceState *st = nullptr;
-    SCIR_TRY(current_device_state(&st));
-    std::lock_guard<std::mutex> lk(st->mu);
-    return vjp_db_run(g, ldg, x, ldx, K, B, N, db, as_stream(stream), st);
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        SCIR_TRY(vjp_db_args(g, ldg, x, ldx, K, B, N, db));
+        if (K == 0) return SCIR_HIP_OK;
+        DeviceState *st = nullptr;
+        SCIR_TRY(current_device_state(&st));
+        std::lock_guard<std::mutex> lk(st->mu);
+        return vjp_db_run(g, ldg, x, ldx, K, B, N, db, as_stream(stream), st);
+    });
 }
 
 int scir_hip_fir_vjp_f32_host(const float *x, int64_t ldx, const float *g, int64_t ldg,
                               const float *b, int32_t K, int64_t B, int64_t N, float *dx,
                               int64_t lddx, float *db, int32_t precision)
 {
-    clear_error();
-    SCIR_TRY(check_dense_args(g, ldg, b, K, B, N, dx, lddx));
-    SCIR_TRY(vjp_db_args(g, ldg, x, ldx, K, B, N, db));
-    if (precision < SCIR_HIP_PREC_F32 || precision > SCIR_HIP_PREC_FFT32)
-        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "unknown precision %d", precision);
-    DeviceState *st = nullptr;
-    SCIR_TRY(current_device_state(&st));
-    std::lock_guard<std::mutex> lk(st->mu);
-    if (!st->host_stream)
-        SCIR_HIP_TRY(hipStreamCreateWithFlags(&st->host_stream, hipStreamNonBlocking));
-    hipStream_t s = st->host_stream;
-    // Device rows padded to a multiple of 4 floats (16-byte aligned rows).
-    const int64_t ldd = (N + 3) & ~int64_t(3);
-    const size_t mat_bytes = (size_t)std::max<int64_t>(B, 1) * ldd * sizeof(float);
-    const size_t row_bytes = (size_t)N * sizeof(float);
-    SCIR_TRY(ensure_buffer(st->host_x, mat_bytes, s));
-    SCIR_TRY(ensure_buffer(st->host_y, mat_bytes, s));
-    SCIR_TRY(ensure_buffer(st->host_z, mat_bytes, s));
-    SCIR_TRY(ensure_buffer(st->host_b, (size_t)std::max(2 * K, 1) * sizeof(float), s));
-    float *dxx = static_cast<float *>(st->host_x.ptr);
-    float *dg = static_cast<float *>(st->host_y.ptr);
-    float *ddx = static_cast<float *>(st->host_z.ptr);
-    float *dbuf = static_cast<float *>(st->host_b.ptr);
-    if (B > 0 && N > 0) {
-        SCIR_HIP_TRY(hipMemcpy2DAsync(dxx, ldd * sizeof(float), x, ldx * sizeof(float), row_bytes,
-                                      B, hipMemcpyHostToDevice, s));
-        SCIR_HIP_TRY(hipMemcpy2DAsync(dg, ldd * sizeof(float), g, ldg * sizeof(float), row_bytes,
-                                      B, hipMemcpyHostToDevice, s));
-    }
-    if (K > 0)
-        SCIR_HIP_TRY(hipMemcpyAsync(dbuf, b, (size_t)K * sizeof(float), hipMemcpyHostToDevice, s));
-    if (B > 0 && N > 0)
-        SCIR_TRY(fir_dense_dispatch(dg, ldd, dbuf, K, B, N, ddx, ldd, precision, true, s, st));
-    if (K > 0) SCIR_TRY(vjp_db_run(dg, ldd, dxx, ldd, K, B, N, dbuf + K, s, st));
-    if (B > 0 && N > 0)
-        SCIR_HIP_TRY(hipMemcpy2DAsync(dx, lddx * sizeof(float), ddx, ldd * sizeof(float), row_bytes,
-                                      B, hipMemcpyDeviceToHost, s));
-    if (K > 0)
-        SCIR_HIP_TRY(hipMemcpyAsync(db, dbuf + K, (size_t)K * sizeof(float),
-                                    hipMemcpyDeviceToHost, s));
-    SCIR_HIP_TRY(hipStreamSynchronize(s));
-    return SCIR_HIP_OK;
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        SCIR_TRY(check_dense_args(g, ldg, b, K, B, N, dx, lddx));
+        SCIR_TRY(vjp_db_args(g, ldg, x, ldx, K, B, N, db));
+        if (precision < SCIR_HIP_PREC_F32 || precision > SCIR_HIP_PREC_FFT32)
+            return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "unknown precision %d", precision);
+        DeviceState *st = nullptr;
+        SCIR_TRY(current_device_state(&st));
+        std::lock_guard<std::mutex> lk(st->mu);
+        if (!st->host_stream)
+            SCIR_HIP_TRY(hipStreamCreateWithFlags(&st->host_stream, hipStreamNonBlocking));
+        hipStream_t s = st->host_stream;
+        // Device rows padded to a multiple of 4 floats (16-byte aligned rows).
+        const int64_t ldd = (N + 3) & ~int64_t(3);
+        const size_t mat_bytes = (size_t)std::max<int64_t>(B, 1) * ldd * sizeof(float);
+        const size_t row_bytes = (size_t)N * sizeof(float);
+        SCIR_TRY(ensure_buffer(st->host_x, mat_bytes, s));
+        SCIR_TRY(ensure_buffer(st->host_y, mat_bytes, s));
+        SCIR_TRY(ensure_buffer(st->host_z, mat_bytes, s));
+        SCIR_TRY(ensure_buffer(st->host_b, (size_t)std::max(2 * K, 1) * sizeof(float), s));
+        float *dxx = static_cast<float *>(st->host_x.ptr);
+        float *dg = static_cast<float *>(st->host_y.ptr);
+        float *ddx = static_cast<float *>(st->host_z.ptr);
+        float *dbuf = static_cast<float *>(st->host_b.ptr);
+        if (B > 0 && N > 0) {
+            SCIR_HIP_TRY(hipMemcpy2DAsync(dxx, ldd * sizeof(float), x, ldx * sizeof(float), row_bytes,
+                                          B, hipMemcpyHostToDevice, s));
+            SCIR_HIP_TRY(hipMemcpy2DAsync(dg, ldd * sizeof(float), g, ldg * sizeof(float), row_bytes,
+                                          B, hipMemcpyHostToDevice, s));
+        }
+        if (K > 0)
+            SCIR_HIP_TRY(hipMemcpyAsync(dbuf, b, (size_t)K * sizeof(float), hipMemcpyHostToDevice, s));
+        if (B > 0 && N > 0)
+            SCIR_TRY(fir_dense_dispatch(dg, ldd, dbuf, K, B, N, ddx, ldd, precision, true, s, st));
+        if (K > 0) SCIR_TRY(vjp_db_run(dg, ldd, dxx, ldd, K, B, N, dbuf + K, s, st));
+        if (B > 0 && N > 0)
+            SCIR_HIP_TRY(hipMemcpy2DAsync(dx, lddx * sizeof(float), ddx, ldd * sizeof(float), row_bytes,
+                                          B, hipMemcpyDeviceToHost, s));
+        if (K > 0)
+            SCIR_HIP_TRY(hipMemcpyAsync(db, dbuf + K, (size_t)K * sizeof(float),
+                                        hipMemcpyDeviceToHost, s));
+        SCIR_HIP_TRY(hipStreamSynchronize(s));
+        return SCIR_HIP_OK;
+    });
 }
 
 }  // extern "C"

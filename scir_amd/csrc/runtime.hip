@@ -18,24 +18,9 @@
 namespace scir {
 
 namespace {
-thread_local std::string t_last_error;
-
 std::mutex g_states_mu;
 std::vector<std::unique_ptr<DeviceState>> g_states;
 }  // namespace
-
-int fail(int code, const char *fmt, ...)
-{
-    char buf[1024];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    t_last_error = buf;
-    return code;
-}
-
-void clear_error() { t_last_error.clear(); }
 
 static int init_state(DeviceState *st, int dev)
 {
@@ -69,15 +54,36 @@ int current_device_state(DeviceState **out)
     return SCIR_HIP_OK;
 }
 
-int ensure_buffer(DeviceBuffer &buf, size_t bytes, hipStream_t stream)
+int device_alloc(void **ptr, size_t bytes, const char *what)
 {
-    if (bytes <= buf.bytes) return SCIR_HIP_OK;
+    *ptr = nullptr;
+    if (alloc_failure_injected())
+        return fail(SCIR_HIP_ERR_OUT_OF_MEMORY, "%s: device allocation of %zu bytes failed "
+                    "(SCIR_HIP_FAIL_ALLOC is set)", what, bytes);
+    const hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(e == hipErrorOutOfMemory ? SCIR_HIP_ERR_OUT_OF_MEMORY : SCIR_HIP_ERR_BACKEND,
+                    "%s: hipMalloc(%zu) failed: %s", what, bytes, hipGetErrorString(e));
+    }
+    return SCIR_HIP_OK;
+}
+
+static int refuse_growth_in_capture(size_t bytes, hipStream_t stream)
+{
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (stream != nullptr && hipStreamIsCapturing(stream, &cs) == hipSuccess &&
         cs != hipStreamCaptureStatusNone)
         return fail(SCIR_HIP_ERR_CAPTURE,
                     "scratch must grow to %zu bytes during stream capture; run the call "
                     "once outside capture first", bytes);
+    return SCIR_HIP_OK;
+}
+
+int ensure_buffer(DeviceBuffer &buf, size_t bytes, hipStream_t stream)
+{
+    if (bytes <= buf.bytes) return SCIR_HIP_OK;
+    SCIR_TRY(refuse_growth_in_capture(bytes, stream));
     if (buf.ptr) {
         // Earlier work on any stream may still read the old block.
         SCIR_HIP_TRY(hipDeviceSynchronize());
@@ -85,8 +91,24 @@ int ensure_buffer(DeviceBuffer &buf, size_t bytes, hipStream_t stream)
         buf.ptr = nullptr;
         buf.bytes = 0;
     }
-    size_t want = bytes < 4096 ? 4096 : bytes;
-    SCIR_HIP_TRY(hipMalloc(&buf.ptr, want));
+    const size_t want = bytes < 4096 ? 4096 : bytes;
+    SCIR_TRY(device_alloc(&buf.ptr, want, "host-tier buffer"));
+    buf.bytes = want;
+    return SCIR_HIP_OK;
+}
+
+int ensure_scratch(DeviceState *st, DeviceBuffer &buf, size_t bytes, hipStream_t stream)
+{
+    if (bytes <= buf.bytes) return SCIR_HIP_OK;
+    SCIR_TRY(refuse_growth_in_capture(bytes, stream));
+    host_alloc_point();
+    st->retired.reserve(st->retired.size() + 1);
+    size_t want = std::max<size_t>(bytes, 2 * buf.bytes);
+    if (want < 4096) want = 4096;
+    void *p = nullptr;
+    SCIR_TRY(device_alloc(&p, want, "library scratch"));
+    if (buf.ptr) st->retired.push_back(buf.ptr);  // reserved above: cannot throw
+    buf.ptr = p;
     buf.bytes = want;
     return SCIR_HIP_OK;
 }
@@ -196,74 +218,93 @@ using namespace scir;
 
 extern "C" {
 
-int scir_hip_version(void) { return 10000; }
-
-const char *scir_hip_last_error(void) { return t_last_error.c_str(); }
-
 int scir_hip_device_count(int *count)
 {
-    if (!count) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "count is null");
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
-    *count = n;
-    return SCIR_HIP_OK;
+    return guarded(__func__, [&]() -> int {
+        if (!count) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "count is null");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        *count = n;
+        return SCIR_HIP_OK;
+    });
 }
 
 int scir_hip_init(void)
 {
-    clear_error();
-    DeviceState *st = nullptr;
-    return current_device_state(&st);
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        DeviceState *st = nullptr;
+        return current_device_state(&st);
+    });
 }
 
 int scir_hip_shutdown(void)
 {
-    clear_error();
-    std::lock_guard<std::mutex> g(g_states_mu);
-    int prev = -1;
-    (void)hipGetDevice(&prev);
-    for (auto &up : g_states) {
-        if (!up || !up->ready) continue;
-        std::lock_guard<std::mutex> lk(up->mu);
-        (void)hipSetDevice(up->device);
-        (void)hipDeviceSynchronize();
-        auto release = [](DeviceBuffer &b) {
-            if (b.ptr) (void)hipFree(b.ptr);
-            b.ptr = nullptr;
-            b.bytes = 0;
-        };
-        for (auto &kv : up->scratch) {
-            release(kv.second.partials);
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        std::lock_guard<std::mutex> g(g_states_mu);
+        int prev = -1;
+        (void)hipGetDevice(&prev);
+        for (auto &up : g_states) {
+            if (!up || !up->ready) continue;
+            std::lock_guard<std::mutex> lk(up->mu);
+            (void)hipSetDevice(up->device);
+            (void)hipDeviceSynchronize();
+            auto release = [](DeviceBuffer &b) {
+                if (b.ptr) (void)hipFree(b.ptr);
+                b.ptr = nullptr;
+                b.bytes = 0;
+            };
+            for (auto &kv : up->scratch) {
+                release(kv.second.partials);
+                release(kv.second.sos_states);
+                release(kv.second.fft_spectrum);
+            }
+            up->scratch.clear();
+            for (void *p : up->retired) (void)hipFree(p);
+            up->retired.clear();
+            for (DeviceBuffer &t : up->fft_twiddles) release(t);
+            release(up->host_x);
+            release(up->host_y);
+            release(up->host_b);
+            release(up->host_z);
+            for (void *&p : up->pinned) {
+                if (p) (void)hipHostFree(p);
+                p = nullptr;
+            }
+            up->pinned_bytes = 0;
+            if (up->host_stream) (void)hipStreamDestroy(up->host_stream);
+            up->host_stream = nullptr;
+            for (hipStream_t &cs : up->host_copy_stream) {
+                if (cs) (void)hipStreamDestroy(cs);
+                cs = nullptr;
+            }
         }
-        up->scratch.clear();
-        release(up->host_x);
-        release(up->host_y);
-        release(up->host_b);
-        if (up->host_stream) (void)hipStreamDestroy(up->host_stream);
-        up->host_stream = nullptr;
-    }
-    if (prev >= 0) (void)hipSetDevice(prev);
-    return SCIR_HIP_OK;
+        if (prev >= 0) (void)hipSetDevice(prev);
+        return SCIR_HIP_OK;
+    });
 }
 
 int scir_hip_synth_normal_f32(float *x, int64_t ldx, int64_t B, int64_t N, uint64_t seed,
                               float outlier_frac, float outlier_scale, void *stream)
 {
-    clear_error();
-    if (B < 0 || N < 0 || ldx < N)
-        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "bad shape for synth (B=%lld N=%lld ldx=%lld)",
-                    (long long)B, (long long)N, (long long)ldx);
-    if (B == 0 || N == 0) return SCIR_HIP_OK;
-    if (!x) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "x is null");
-    DeviceState *st = nullptr;
-    SCIR_TRY(current_device_state(&st));
-    int64_t total = B * N;
-    int64_t blocks = (total + 255) / 256;
-    int64_t cap = (int64_t)st->cu_count * 64;
-    if (blocks > cap) blocks = cap;
-    hipLaunchKernelGGL(synth_normal_kernel, dim3((unsigned)blocks), dim3(256), 0,
-                       as_stream(stream), x, ldx, N, total, seed, outlier_frac, outlier_scale);
-    return check_launch("synth_normal_kernel");
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        if (B < 0 || N < 0 || ldx < N)
+            return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "bad shape for synth (B=%lld N=%lld ldx=%lld)",
+                        (long long)B, (long long)N, (long long)ldx);
+        if (B == 0 || N == 0) return SCIR_HIP_OK;
+        if (!x) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "x is null");
+        DeviceState *st = nullptr;
+        SCIR_TRY(current_device_state(&st));
+        int64_t total = B * N;
+        int64_t blocks = (total + 255) / 256;
+        int64_t cap = (int64_t)st->cu_count * 64;
+        if (blocks > cap) blocks = cap;
+        hipLaunchKernelGGL(synth_normal_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                           as_stream(stream), x, ldx, N, total, seed, outlier_frac, outlier_scale);
+        return check_launch("synth_normal_kernel");
+    });
 }
 
 // ---- HIP graphs -------------------------------------------------------------
@@ -275,51 +316,67 @@ struct scir_hip_graph_s {
 
 int scir_hip_graph_begin(void *stream)
 {
-    clear_error();
-    if (!stream)
-        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "graph capture needs a non-null stream");
-    DeviceState *st = nullptr;
-    SCIR_TRY(current_device_state(&st));
-    SCIR_HIP_TRY(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal));
-    return SCIR_HIP_OK;
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        if (!stream)
+            return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "graph capture needs a non-null stream");
+        DeviceState *st = nullptr;
+        SCIR_TRY(current_device_state(&st));
+        SCIR_HIP_TRY(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal));
+        return SCIR_HIP_OK;
+    });
 }
 
 int scir_hip_graph_end(void *stream, scir_hip_graph *graph)
 {
-    clear_error();
-    if (!stream || !graph)
-        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "null stream or graph handle");
-    hipGraph_t g = nullptr;
-    SCIR_HIP_TRY(hipStreamEndCapture(as_stream(stream), &g));
-    hipGraphExec_t ex = nullptr;
-    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-        (void)hipGraphDestroy(g);
-        return fail(SCIR_HIP_ERR_BACKEND, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
-    }
-    auto *h = new scir_hip_graph_s;
-    h->graph = g;
-    h->exec = ex;
-    *graph = h;
-    return SCIR_HIP_OK;
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        if (!stream || !graph)
+            return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "null stream or graph handle");
+        *graph = nullptr;
+        hipGraph_t g = nullptr;
+        SCIR_HIP_TRY(hipStreamEndCapture(as_stream(stream), &g));
+        std::unique_ptr<scir_hip_graph_s> h;
+        try {
+            host_alloc_point();
+            h.reset(new scir_hip_graph_s);
+        } catch (...) {
+            (void)hipGraphDestroy(g);
+            throw;
+        }
+        hipGraphExec_t ex = nullptr;
+        hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return fail(SCIR_HIP_ERR_BACKEND, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        }
+        h->graph = g;
+        h->exec = ex;
+        *graph = h.release();
+        return SCIR_HIP_OK;
+    });
 }
 
 int scir_hip_graph_launch(scir_hip_graph graph, void *stream)
 {
-    clear_error();
-    if (!graph) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "graph is null");
-    SCIR_HIP_TRY(hipGraphLaunch(graph->exec, as_stream(stream)));
-    return SCIR_HIP_OK;
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        if (!graph) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "graph is null");
+        SCIR_HIP_TRY(hipGraphLaunch(graph->exec, as_stream(stream)));
+        return SCIR_HIP_OK;
+    });
 }
 
 int scir_hip_graph_destroy(scir_hip_graph graph)
 {
-    clear_error();
-    if (!graph) return SCIR_HIP_OK;
-    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
-    if (graph->graph) (void)hipGraphDestroy(graph->graph);
-    delete graph;
-    return SCIR_HIP_OK;
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        if (!graph) return SCIR_HIP_OK;
+        if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+        if (graph->graph) (void)hipGraphDestroy(graph->graph);
+        delete graph;
+        return SCIR_HIP_OK;
+    });
 }
 
 }  // extern "C"

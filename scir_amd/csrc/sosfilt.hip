@@ -694,7 +694,7 @@ int sos_run(const double *x, int64_t ldx, double *y, int64_t ldy, int64_t rows, 
         const int Dk = 2 * (S <= 8 ? S : kMaxSections);  // the kernels' padded dimension
         const int64_t mdoubles = (mats_size(Dk, d) + 31) / 32 * 32;
         const int64_t zdoubles = nC > 1 ? rows * nC * d : 0;
-        SCIR_TRY(ensure_buffer(sc.sos_states, (size_t)(mdoubles + zdoubles) * sizeof(double), s));
+        SCIR_TRY(ensure_scratch(st, sc.sos_states, (size_t)(mdoubles + zdoubles) * sizeof(double), s));
         mats = static_cast<double *>(sc.sos_states.ptr);
         zs = mats + mdoubles;
     }
@@ -783,27 +783,35 @@ int scir_hip_sosfilt_f64_dev(const double *x, int64_t ldx, int64_t rows, int64_t
                              const double *sos, int32_t S, double *y, int64_t ldy, int32_t flags,
                              void *stream)
 {
-    return sos_dev(x, ldx, rows, n, sos, S, y, ldy, flags, false, stream);
+    return guarded(__func__, [&]() -> int {
+        return sos_dev(x, ldx, rows, n, sos, S, y, ldy, flags, false, stream);
+    });
 }
 
 int scir_hip_filtfilt_f64_dev(const double *x, int64_t ldx, int64_t rows, int64_t n,
                               const double *sos, int32_t S, double *y, int64_t ldy, int32_t flags,
                               void *stream)
 {
-    return sos_dev(x, ldx, rows, n, sos, S, y, ldy, flags, true, stream);
+    return guarded(__func__, [&]() -> int {
+        return sos_dev(x, ldx, rows, n, sos, S, y, ldy, flags, true, stream);
+    });
 }
 
 int scir_hip_sosfilt_f64_host(const double *x, int64_t ldx, int64_t rows, int64_t n,
                               const double *sos, int32_t S, double *y, int64_t ldy, int32_t flags)
 {
-    return sos_host(x, ldx, rows, n, sos, S, y, ldy, flags, false);
+    return guarded(__func__, [&]() -> int {
+        return sos_host(x, ldx, rows, n, sos, S, y, ldy, flags, false);
+    });
 }
 
 int scir_hip_filtfilt_f64_host(const double *x, int64_t ldx, int64_t rows, int64_t n,
                                const double *sos, int32_t S, double *y, int64_t ldy,
                                int32_t flags)
 {
-    return sos_host(x, ldx, rows, n, sos, S, y, ldy, flags, true);
+    return guarded(__func__, [&]() -> int {
+        return sos_host(x, ldx, rows, n, sos, S, y, ldy, flags, true);
+    });
 }
 
 }  // extern "C"

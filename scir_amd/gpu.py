@@ -97,13 +97,15 @@ def fir1d_batched_f32(x, taps) -> np.ndarray:
     return y
 
 
-def fir1d_batched_f32_hip(x, taps, precision="f32") -> np.ndarray:
+def fir1d_batched_f32_hip(x, taps, precision="auto") -> np.ndarray:
     """HIP batched FIR, host arrays in and out (replaces
     ``cuda::fir1d_batched_f32_cuda``).  Raises ``GpuError`` on any failure.
 
-    ``precision``: ``"f32"`` (default; FMA direct form within the K*eps bound),
-    ``"f32_exact"`` (bit-identical to the CPU loop), ``"bf16x3"``, ``"fp16"``
-    or ``"auto"``."""
+    ``precision``: ``"auto"`` (default: the FP32 direct form for short
+    filters, BF16x3 on MFMA for long ones; both within (K+1)*2^-23*sum|b||x|
+    of the exact sum and non-finite exactly where the CPU loop is),
+    ``"f32"`` (FMA direct form), ``"f32_exact"`` (bit-identical to the CPU
+    loop), ``"bf16x3"``, ``"fp16"`` or ``"fft32"``."""
     xa = _as_x(x)
     b = _as_b(taps)
     B, N = xa.shape

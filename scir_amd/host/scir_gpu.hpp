@@ -128,10 +128,12 @@ inline Array2<float> fir1d_batched_f32(const ArrayView2<float> &x, const std::ve
     return y;
 }
 
-// HIP FIR: host arrays in and out; throws GpuError on any failure.
+// HIP FIR: host arrays in and out; throws GpuError on any failure.  AUTO
+// (the default) takes the FP32 direct form for short filters and BF16x3 on
+// MFMA for long ones, both within the FP32 bound of the reference loop.
 inline Array2<float> fir1d_batched_f32_hip(const ArrayView2<float> &x,
                                            const std::vector<float> &taps,
-                                           scir_hip_precision precision = SCIR_HIP_PREC_F32)
+                                           scir_hip_precision precision = SCIR_HIP_PREC_AUTO)
 {
     Array2<float> xs = detail::standard(x);
     std::vector<float> b = detail::reversed(taps);

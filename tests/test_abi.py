@@ -43,7 +43,7 @@ def test_binding_covers_header():
 
 def test_version_and_device_count():
     L = _lib.lib()
-    assert L.scir_hip_version() == 10000
+    assert L.scir_hip_version() == 10100
     assert _lib.device_count() >= 0
 
 

@@ -52,3 +52,60 @@ def test_overrides_validated():
     assert L.scir_hip_tune_set(b"no.such.key", 1) == _lib.ERR_INVALID_ARGUMENT
     assert L.scir_hip_tune_set(b"direct.r", 16) == _lib.OK
     assert L.scir_hip_tune_set(b"direct.r", -1) == _lib.OK
+
+
+RAGGED = """{"arch": "gfx950", "auto_mfma_min_k": 128, "auto_fft_min_k": 384,
+ "ragged": {"direct_wg_per_cu": 4, "direct_min_item_kmacs": 2048, "mfma_segs_per_item": 16,
+            "mfma_wg_per_cu": 1},
+ "entries": []}
+"""
+
+
+def test_load_table_with_ragged_shapes(tmp_path):
+    p = tmp_path / "tune.json"
+    p.write_text(RAGGED)
+    L = _lib.lib()
+    assert L.scir_hip_autotune_load(str(p).encode()) == _lib.OK
+    p.write_text('{"arch": "gfx950", "ragged": {"direct_wg_per_cu": 4, "entries": []')
+    assert L.scir_hip_autotune_load(str(p).encode()) == _lib.ERR_INVALID_ARGUMENT
+    assert b'malformed table ("ragged")' in L.scir_hip_last_error()
+
+
+def test_malformed_message_names_the_missing_key(tmp_path):
+    L = _lib.lib()
+    p = tmp_path / "t.json"
+    p.write_text('{"arch": "gfx950", "entries": [{"k_lo": 1}]}')
+    assert L.scir_hip_autotune_load(str(p).encode()) == _lib.ERR_INVALID_ARGUMENT
+    assert b'entry without "k_hi"' in L.scir_hip_last_error()
+    p.write_text('{"arch": "gfx950", "entries": [{"k_lo": 9, "k_hi": 2, "n_lo": 0, "n_hi": 1, '
+                 '"direct_r": 8, "direct_loader": 1, "direct_seg_tiles": 2, "mfma_tiles": 2, '
+                 '"mfma_tiles_fp16": 4}]}')
+    assert L.scir_hip_autotune_load(str(p).encode()) == _lib.ERR_INVALID_ARGUMENT
+    assert b"empty bucket" in L.scir_hip_last_error()
+
+
+def test_truncated_tables_never_crash(tmp_path):
+    """Every prefix of a valid table either loads or is rejected with a
+    message (the reader never reads past the text)."""
+    L = _lib.lib()
+    p = tmp_path / "t.json"
+    for cut in range(0, len(TABLE), 7):
+        p.write_text(TABLE[:cut])
+        st = L.scir_hip_autotune_load(str(p).encode())
+        assert st in (_lib.OK, _lib.ERR_INVALID_ARGUMENT)
+        if st != _lib.OK:
+            assert L.scir_hip_last_error()
+    p.write_text(TABLE)
+    assert L.scir_hip_autotune_load(str(p).encode()) == _lib.OK
+
+
+def test_ragged_overrides_validated():
+    L = _lib.lib()
+    for key, bad, good in ((b"ragged.direct_wg_per_cu", 0, 8), (b"ragged.direct_wg_per_cu", 33, 4),
+                           (b"ragged.direct_min_item_kmacs", 0, 256),
+                           (b"ragged.mfma_segs_per_item", 0, 4),
+                           (b"ragged.mfma_segs_per_item", 2000, 32),
+                           (b"ragged.mfma_wg_per_cu", 9, 1)):
+        assert L.scir_hip_tune_set(key, bad) == _lib.ERR_INVALID_ARGUMENT
+        assert L.scir_hip_tune_set(key, good) == _lib.OK
+        assert L.scir_hip_tune_set(key, -1) == _lib.OK

@@ -170,6 +170,28 @@ def test_full_size_bandwidth_workload(hip, oracle, K):
     assert np.all(np.abs(y[rows].cpu().numpy() - y64) <= oracle.fir_error_bound(xs, b_np))
 
 
+@pytest.mark.parametrize("K", [3, 21, 300, 1023])
+def test_signal_forwarder_hip_device(hip, oracle, K):
+    """scir_signal::gpu::fir1d_batched_f32 (crates/scir-signal/src/lib.rs:371-374,
+    its test :670-696) on Device::Hip: the forwarder reaches the HIP arm (AUTO:
+    the direct form for short filters, BF16x3 on MFMA for long ones) and stays
+    within the FP32 bound of the reference loop; the reference's 2x4 case
+    gives its known answer exactly."""
+    from scir_amd import Device, signal
+    x = np.array([[1.0, 2.0, 3.0, 4.0], [0.5, 0.0, -0.5, -1.0]], np.float32)
+    kat = signal.fir1d_batched_f32(x, np.array([0.25, 0.5, 0.25], np.float32), Device.Hip)
+    np.testing.assert_array_equal(kat, [[0.25, 1.0, 2.0, 3.0], [0.125, 0.25, 0.0, -0.5]])
+    r = rng(900 + K)
+    x = r.standard_normal((5, 6000)).astype(np.float32)
+    taps = (r.standard_normal(K) / np.sqrt(K)).astype(np.float32)
+    y = signal.fir1d_batched_f32(x, taps, Device.Hip)
+    assert y.shape == x.shape and y.dtype == np.float32
+    y64 = oracle.fir_ref_f64(x.astype(np.float64), taps.astype(np.float64))
+    assert np.all(np.abs(y - y64) <= oracle.fir_error_bound(x, taps[::-1]))
+    y_cpu = signal.fir1d_batched_f32(x, taps, Device.Cpu)
+    np.testing.assert_array_equal(y_cpu, oracle.fir_ref_f32(x, taps))
+
+
 def test_auto_dispatch_hip_device(hip, oracle):
     import scir_amd
     r = rng(8)

@@ -129,3 +129,55 @@ def test_sosfilt_and_resample_graph_replays(hip):
         assert torch.equal(z, ze), f"filtfilt replay {rep} differs"
         assert torch.equal(w, we), f"resample replay {rep} differs"
     gr.close()
+
+
+def test_graph_survives_scratch_growth_on_its_stream(hip):
+    """A graph holds the addresses of the stream's scratch (the FFT filter
+    spectrum, the dL/db partials).  An eager call with a longer filter on the
+    same stream grows that scratch; the library keeps the outgrown blocks, so
+    later replays still read and write valid memory and equal eager results
+    bit for bit."""
+    import torch
+    from scir_amd import device
+    from scir_amd.ragged import Graph
+    r = rng(4242)
+    B, N, K = 4, 60000, 512
+    x = torch.tensor(r.standard_normal((B, N)).astype(np.float32), device="cuda")
+    g = torch.tensor(r.standard_normal((B, N)).astype(np.float32), device="cuda")
+    b = torch.tensor(r.standard_normal(K).astype(np.float32), device="cuda")
+    s = torch.cuda.Stream()
+    y, db = torch.empty_like(x), torch.empty(K, device="cuda")
+    with torch.cuda.stream(s):  # scratch sized for K = 512 outside the capture
+        device.fir(x, b, "fft32", out=y)
+        device.vjp_db(g, x, K, out=db)
+    s.synchronize()
+    with Graph(s) as gr:
+        device.fir(x, b, "fft32", out=y)
+        device.vjp_db(g, x, K, out=db)
+    # grow both scratch blocks on the capture stream (K = 2048: twice the
+    # spectrum, four times the partials)
+    K2 = 2048
+    b2 = torch.tensor(r.standard_normal(K2).astype(np.float32), device="cuda")
+    with torch.cuda.stream(s):
+        y2 = device.fir(x, b2, "fft32")
+        db2 = device.vjp_db(g, x, K2)
+    s.synchronize()
+    for rep in range(3):
+        x.mul_(-1.25)
+        ye = device.fir(x, b, "fft32")
+        dbe = device.vjp_db(g, x, K)
+        y.fill_(float("nan"))
+        db.fill_(float("nan"))
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            gr.launch()
+            # the larger eager calls between replays keep using the new blocks
+            y2b = device.fir(x, b2, "fft32")
+            db2b = device.vjp_db(g, x, K2)
+        s.synchronize()
+        assert torch.equal(y, ye), f"forward replay {rep} differs after scratch growth"
+        assert torch.equal(db, dbe), f"dL/db replay {rep} differs after scratch growth"
+        assert torch.equal(y2b, device.fir(x, b2, "fft32"))
+        assert torch.equal(db2b, device.vjp_db(g, x, K2))
+    gr.close()
+    del y2, db2

@@ -45,6 +45,21 @@ def test_cpu_arm_bitwise_vs_oracle(oracle, B, N, K):
         np.testing.assert_array_equal(y, oracle.fir_ref_f32(x, taps))
 
 
+@pytest.mark.parametrize("K", [1, 4, 33])
+def test_cpu_arm_nonfinite_bitwise_vs_oracle(oracle, K):
+    """Inf and NaN in x and taps: the CPU arm forms only the reference
+    loop's terms, so it equals the oracle NaN for NaN."""
+    r = rng(60 + K)
+    x = r.standard_normal((3, 200)).astype(np.float32)
+    taps = r.standard_normal(K).astype(np.float32)
+    x[0, 3], x[1, 100], x[2, 150] = np.inf, -np.inf, np.nan
+    taps[0] = np.inf
+    if K > 2:
+        taps[1] = np.nan
+    y = scir_amd.fir1d_batched_f32(x, taps)
+    assert np.array_equal(y, oracle.fir_ref_f32(x, taps), equal_nan=True)
+
+
 def test_cpu_config_workload_cpu_arm(oracle):
     """The reference's CPU case (x [64, 16384] ~ N(0,1) f32, 33-tap Hamming
     low-pass, cutoff 0.2 of Nyquist) on the library's CPU arm: bit for bit

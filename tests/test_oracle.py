@@ -63,6 +63,56 @@ def test_planted_fma_is_caught(oracle):
     assert np.any(fused[0] != ref[0])
 
 
+def poison(x, taps):
+    """+Inf, -Inf and NaN in x and in the taps (the first taps in the
+    reference's order, i.e. the last b[k], are met only by outputs i >= K-1)."""
+    x = x.copy()
+    taps = taps.copy()
+    x[0, 5] = np.inf
+    x[1, 40] = -np.inf
+    x[2, 70] = np.nan
+    x[0, 80] = np.inf
+    x[0, 81] = -np.inf
+    taps[0] = np.inf
+    taps[2] = np.nan
+    taps[-1] = -np.inf
+    return x, taps
+
+
+@pytest.mark.parametrize("K", [1, 3, 8, 31])
+def test_f32_nonfinite_bitwise_vs_numpy_emulation(oracle, K):
+    """Inf and NaN reach exactly the outputs whose reference terms hold them:
+    the oracle equals the emulation NaN for NaN."""
+    r = rng(50 + K)
+    x, taps = poison(r.standard_normal((3, 97)).astype(np.float32),
+                     r.standard_normal(max(K, 3)).astype(np.float32))
+    taps = taps[:K]
+    with np.errstate(invalid="ignore", over="ignore"):
+        ref = numpy_ref_f32(x, taps)
+    y = oracle.fir_ref_f32(x, taps)
+    assert np.array_equal(y, ref, equal_nan=True)
+    assert np.isnan(y).any() and np.isinf(y).any()
+
+
+def test_planted_ramp_in_term_is_caught(oracle):
+    """A restatement that also forms the terms before the row start
+    (taps * 0 for t > i, finite for finite taps) differs from the oracle as
+    soon as a tap is infinite, so the non-finite test above can fail."""
+    r = rng(9)
+    x = r.standard_normal((1, 40)).astype(np.float32)
+    taps = r.standard_normal(6).astype(np.float32)
+    taps[0] = np.inf  # applied to x[i-5]: before the row start for i < 5
+    K = taps.size
+    padded = np.concatenate([np.zeros((1, K - 1), np.float32), x], axis=1)
+    with np.errstate(invalid="ignore"):
+        wrong = np.zeros_like(x)
+        for t in range(K):
+            prod = (taps[K - 1 - t] * padded[:, K - 1 - t:K - 1 - t + 40]).astype(np.float32)
+            wrong = (wrong + prod).astype(np.float32)
+    y = oracle.fir_ref_f32(x, taps)
+    assert np.isnan(wrong[0, :5]).all() and np.isfinite(y[0, :5]).all()
+
+
 def test_impulse_step_dc_closed_forms(oracle):
     K = 7
     taps = rng(1).standard_normal(K).astype(np.float32)
