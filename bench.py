@@ -264,13 +264,19 @@ def roofline_for(wl, name, prec, avg_launch_s):
         ach = alg / avg_launch_s / 1e12
         prods = 6 if prec == "bf16x3" else 1
         issued = 32.0 * mfma_steps(wl["K"]) * prods * samples
+        # Gsamples/s with every MFMA pipe issuing back to back at the nominal
+        # peak: the ceiling of this arithmetic at this K
+        ceiling = BF16_PEAK_TFLOPS * 1e12 / (issued / samples) / 1e9
         return dict(bound="mfma", achieved=round(ach, 2), peak=BF16_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=round(ach / BF16_PEAK_TFLOPS, 4),
                     traffic=load_traffic(f"fir_mfma_{prec}_{name}_pmc"),
                     mfma_issued_tflops=round(issued / avg_launch_s / 1e12, 1),
                     mfma_issued_frac=round(issued / avg_launch_s / 1e12 / BF16_PEAK_TFLOPS, 4),
                     mfma_busy_frac=load_profile(f"fir_mfma_{prec}_{name}_util", "mfma_busy_frac"),
-                    sclk_ghz=load_profile(f"fir_mfma_{prec}_{name}_util", "sclk_ghz"))
+                    sclk_ghz=load_profile(f"fir_mfma_{prec}_{name}_util", "sclk_ghz"),
+                    ceiling_gsamples_s=round(ceiling, 1),
+                    ceiling_note=f"{prods} MFMA products x {mfma_steps(wl['K'])} half-steps of "
+                                 "32x32x16 per 32x32 output tile at 2.5 PFLOP/s")
     flops = 2.0 * wl["K"] * samples
     if prec == "fft32" or flops / (8.0 * samples) < 78.6 / 8.0:
         # HBM-bound (the FFT path's work per sample is O(log K))
@@ -343,7 +349,58 @@ def run_dense(args):
                                              min(5.0, args.cpu_budget))
         if not args.quick:
             out["reference_cpu_case"] = reference_cpu_case(stream)
+    if name == HEADLINE and not args.quick:
+        del x, y
+        torch.cuda.empty_cache()
+        out["configs"] = other_configs(stream)
     return out
+
+
+def other_configs(stream):
+    """Every other configuration of the benchmark set, measured in the same
+    run as the headline, each with its roofline (HBM traffic from the
+    committed counter summaries where one exists):
+      c1_direct_k129    x [4096, 65536], 129-tap low-pass, FP32 direct form
+      c2_longtap_k2048  x [2048, 262144] with 1 % outliers x1e3, 2048-tap
+                        Kaiser band-pass: BF16x3, FP32 direct, FFT32, FP16
+      c3_ragged         10000 CSR signals (one persistent launch, its HIP-graph
+                        replay and 10000 per-signal launches; F32 and BF16x3)
+      c4_vjp_k255       forward + dL/dx + dL/db on x [1024, 65536], K = 255
+    (c1 at K = 33 is ``bandwidth_workload``; c2 at K = 512 is the headline.)"""
+    import torch
+    from scir_amd import device
+    from oracle import oracle as orc
+    res = {}
+    # c1, K = 129: FP32-vector bound
+    wl, x1, b1, b1_np = dense_setup("direct_k129")
+    y1 = torch.empty_like(x1)
+    ms, _ = timed(lambda: device.fir(x1, b1, "f32", out=y1), 20, 30, stream)
+    res["c1_direct_k129"] = dict(
+        rows=wl["B"], samples_per_row=wl["N"], taps=wl["K"], precision="f32",
+        gsamples_s=round(wl["B"] * wl["N"] / (ms * 1e-3) / 1e9, 2), ms_per_step=round(ms, 4),
+        rel_l2_vs_fp64=rel_l2_rows(x1, y1, b1_np, [0, 2048, 4095]),
+        roofline=roofline_for(wl, "direct_k129", "f32", ms * 1e-3))
+    del x1, y1
+    # c2, K = 2048: every arithmetic on the same inputs
+    wl, x2, b2, b2_np = dense_setup("longtap_k2048")
+    y2 = torch.empty_like(x2)
+    rows = [0, 1023, 2047]
+    ref = orc.lfilter_f64(x2[rows].cpu().numpy(), b2_np)
+    c2 = {}
+    for prec, steps in (("bf16x3", 10), ("f32", 4), ("fft32", 10), ("fp16", 10)):
+        ms, _ = timed(lambda: device.fir(x2, b2, prec, out=y2), steps, 5, stream)
+        c2["fp32_direct" if prec == "f32" else prec] = dict(
+            gsamples_s=round(wl["B"] * wl["N"] / (ms * 1e-3) / 1e9, 2), ms_per_step=round(ms, 4),
+            rel_l2_vs_fp64=orc.rel_l2(y2[rows].cpu().numpy(), ref),
+            roofline=roofline_for(wl, "longtap_k2048", prec, ms * 1e-3))
+    res["c2_longtap_k2048"] = dict(rows=wl["B"], samples_per_row=wl["N"], taps=wl["K"],
+                                   rel_l2_rows=rows, **c2)
+    del x2, y2
+    torch.cuda.empty_cache()
+    res["c3_ragged"] = ragged_results(steps=5, warmup=2, precs=("bf16x3", "f32"), stream=stream)
+    torch.cuda.empty_cache()
+    res["c4_vjp_k255"] = vjp_results(steps=20, warmup=10, prec="bf16x3", stream=stream)
+    return res
 
 
 def ragged_rel_l2(x, y, offs, boffs, taps, n=8):
@@ -380,7 +437,11 @@ def ragged_roofline(prec, macs, issued, ms):
                 traffic=load_traffic(f"fir_ragged_{prec}_ragged_pmc"))
 
 
-def run_ragged(args):
+def ragged_results(steps, warmup, precs, stream):
+    """The ragged workload (10000 CSR signals, lengths log-uniform in
+    [256, 2^20], K uniform in [15, 1023]): per precision one persistent
+    launch (timed over `steps`), its HIP-graph replay, and the baseline of one
+    dense launch per signal; rel-L2 vs FP64 on a seeded subset of signals."""
     import torch
     from scir_amd import configs, device
     from scir_amd.ragged import Graph, RaggedPlan
@@ -389,57 +450,66 @@ def run_ragged(args):
     x = device.synth_normal(1, total, seed=configs.SEED_BASE + 3).view(-1)
     bt = torch.tensor(taps, device="cuda")
     y = torch.empty_like(x)
-    prec = args.precision or "f32"
-    if prec not in ("f32", "f32_exact", "bf16x3", "fp16"):
-        raise SystemExit("--precision for ragged: f32, f32_exact, bf16x3 or fp16")
-    plan = RaggedPlan(offs, boffs, prec)
-    stream = torch.cuda.current_stream()
-    ms, wall = timed(lambda: plan.execute(x, bt, y), args.steps, args.warmup, stream)
     lens, ks = np.diff(offs), np.diff(boffs)
     macs = float(np.sum(lens * ks))
     # MFMA flops issued: 32x32x16 blocks of 8192-output segments, per product.
     segs = (lens + 8191) // 8192
     issued = float(np.sum(segs * 8192 * 32.0 * np.array([mfma_steps(int(k)) for k in ks])))
-    err, nsig = ragged_rel_l2(x, y, offs, boffs, taps)
-    out = dict(metric=METRIC, value=round(total / (ms * 1e-3) / 1e9, 2), unit="Gsamples/s",
-               n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 4),
+    res = dict(signals=int(offs.size - 1), samples=total, lengths="log-uniform [256, 2^20]",
+               taps="uniform [15, 1023], random Hamming low-pass")
+    for prec in precs:
+        plan = RaggedPlan(offs, boffs, prec)
+        ms, _ = timed(lambda: plan.execute(x, bt, y), steps, warmup, stream)
+        err, nsig = ragged_rel_l2(x, y, offs, boffs, taps)
+        ms_s, _ = timed(lambda: plan.execute_per_signal(x, bt, y), 1, 1, stream)
+        side = torch.cuda.Stream()
+        torch.cuda.synchronize()
+        with Graph(side) as g:
+            plan.execute(x, bt, y)
+        ms_g, _ = timed(g.launch, max(3, steps // 2), 1, side)
+        g.close()
+        plan.close()
+        res[prec] = dict(gsamples_s=round(total / (ms * 1e-3) / 1e9, 2), ms_per_step=round(ms, 4),
+                         rel_l2_vs_fp64=err, rel_l2_signals=nsig,
+                         graph_replay_ms=round(ms_g, 4), per_signal_launches_ms=round(ms_s, 3),
+                         persistent_speedup=round(ms_s / ms, 2),
+                         roofline=ragged_roofline(prec, macs, issued, ms))
+    del x, y
+    return res
+
+
+def run_ragged(args):
+    from scir_amd import configs
+    prec = args.precision or "f32"
+    if prec not in ("f32", "f32_exact", "bf16x3", "fp16"):
+        raise SystemExit("--precision for ragged: f32, f32_exact, bf16x3 or fp16")
+    import torch
+    stream = torch.cuda.current_stream()
+    precs = (prec,) if args.quick else (prec,) + tuple(p for p in ("f32", "bf16x3") if p != prec)
+    t0 = time.perf_counter()
+    r = ragged_results(args.steps, args.warmup, precs, stream)
+    wall = time.perf_counter() - t0
+    main_r = r[prec]
+    offs, _, _ = configs.ragged_layout()
+    out = dict(metric=METRIC, value=main_r["gsamples_s"], unit="Gsamples/s",
+               n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=main_r["ms_per_step"],
                higher_is_better=True, scaling="weak", vs_baseline=None, dtype=prec,
                data="synthetic",
-               config=dict(workload="ragged", signals=offs.size - 1, samples=total,
-                           lengths="log-uniform [256, 2^20]", taps="uniform [15, 1023]",
-                           precision=prec),
-               rel_l2_vs_fp64=err, rel_l2_signals=nsig, wall_s=round(wall, 3),
-               roofline=ragged_roofline(prec, macs, issued, ms))
-    if args.quick:
-        plan.close()
-        return out
-    ms_s, _ = timed(lambda: plan.execute_per_signal(x, bt, y), 1, 1, stream)
-    side = torch.cuda.Stream()
-    torch.cuda.synchronize()
-    with Graph(side) as g:
-        plan.execute(x, bt, y)
-
-    def replay():
-        g.launch()
-    ms_g, _ = timed(replay, max(3, args.steps // 2), 1, side)
-    out["compare"] = dict(per_signal_launches_ms=round(ms_s, 3), graph_replay_ms=round(ms_g, 3),
-                          persistent_speedup=round(ms_s / ms, 2))
-    g.close()
-    plan.close()
-    for other in ("f32", "bf16x3"):
-        if other == prec:
-            continue
-        p2 = RaggedPlan(offs, boffs, other)
-        ms_o, _ = timed(lambda: p2.execute(x, bt, y), max(3, args.steps // 2), 2, stream)
-        e2, _ = ragged_rel_l2(x, y, offs, boffs, taps)
-        out["compare"][other] = dict(gsamples_s=round(total / (ms_o * 1e-3) / 1e9, 2),
-                                     ms_per_step=round(ms_o, 4), rel_l2_vs_fp64=e2,
-                                     roofline=ragged_roofline(other, macs, issued, ms_o))
-        p2.close()
+               config=dict(workload="ragged", signals=offs.size - 1, samples=r["samples"],
+                           lengths=r["lengths"], taps=r["taps"], precision=prec),
+               rel_l2_vs_fp64=main_r["rel_l2_vs_fp64"], rel_l2_signals=main_r["rel_l2_signals"],
+               wall_s=round(wall, 3), roofline=main_r["roofline"],
+               compare=dict(per_signal_launches_ms=main_r["per_signal_launches_ms"],
+                            graph_replay_ms=main_r["graph_replay_ms"],
+                            persistent_speedup=main_r["persistent_speedup"],
+                            **{p: r[p] for p in precs if p != prec}))
     return out
 
 
-def run_vjp(args):
+def vjp_results(steps, warmup, prec, stream):
+    """Forward + dL/dx + dL/db for x [1024, 65536], K = 255 low-pass,
+    L = sum(w * y): Gsamples/s of the whole step, and the dL/db kernel
+    (FP32 MFMA) against the FP32 MFMA peak."""
     import torch
     from scir_amd import design, device
     B, N, K = 1024, 65536, 255
@@ -448,30 +518,38 @@ def run_vjp(args):
     b = torch.tensor(design.lowpass(K), device="cuda")
     y, dx = torch.empty_like(x), torch.empty_like(x)
     db = torch.empty(K, device="cuda")
-    prec = args.precision or "bf16x3"
 
     def step():
         device.fir(x, b, prec, out=y)
         device.vjp_dx(w, b, prec, out=dx)
         device.vjp_db(w, x, K, out=db)
-    stream = torch.cuda.current_stream()
-    ms, wall = timed(step, args.steps, args.warmup, stream)
+    ms, wall = timed(step, steps, warmup, stream)
     ms_db, _ = timed(lambda: device.vjp_db(w, x, K, out=db), 5, 1, stream)
     flops_db = 2.0 * K * B * N
     ach = flops_db / (ms_db * 1e-3) / 1e12
-    return dict(metric=METRIC, value=round(B * N / (ms * 1e-3) / 1e9, 2), unit="Gsamples/s",
-                n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 4),
-                higher_is_better=True, scaling="weak", vs_baseline=None, dtype=prec,
-                data="synthetic",
-                config=dict(workload="vjp_k255", rows=B, samples_per_row=N, taps=K,
-                            step="forward + dL/dx + dL/db", precision=prec),
-                wall_s=round(wall, 3),
+    return dict(rows=B, samples_per_row=N, taps=K, step="forward + dL/dx + dL/db",
+                precision=prec, gsamples_s=round(B * N / (ms * 1e-3) / 1e9, 2),
+                ms_per_step=round(ms, 4), wall_s=round(wall, 3),
+                db_ms=round(ms_db, 4),
                 roofline=dict(bound="fp32", achieved=round(ach, 2), peak=FP32_PEAK_TFLOPS,
                               unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 4),
                               kernel="fir_vjp_db_mfma_kernel (FP32 MFMA)",
                               traffic=load_traffic(f"fir_vjp_db_{prec}_vjp_k255_pmc"),
                               mfma_busy_frac=load_profile(f"fir_vjp_db_{prec}_vjp_k255_util",
                                                           "mfma_busy_frac")))
+
+
+def run_vjp(args):
+    import torch
+    prec = args.precision or "bf16x3"
+    r = vjp_results(args.steps, args.warmup, prec, torch.cuda.current_stream())
+    return dict(metric=METRIC, value=r["gsamples_s"], unit="Gsamples/s",
+                n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=r["ms_per_step"],
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype=prec,
+                data="synthetic",
+                config=dict(workload="vjp_k255", rows=r["rows"], samples_per_row=r["samples_per_row"],
+                            taps=r["taps"], step=r["step"], precision=prec),
+                wall_s=r["wall_s"], roofline=r["roofline"])
 
 
 def run_resample(args):
