@@ -289,6 +289,22 @@ SCIR_HIP_API int scir_hip_fir_vjp_dx_dev(const float *g, int64_t ldg, const floa
  * per-tile FP32 partial sums, then a fixed-order compensated sum. */
 SCIR_HIP_API int scir_hip_fir_vjp_db_dev(const float *g, int64_t ldg, const float *x, int64_t ldx,
                             int32_t K, int64_t B, int64_t N, float *db, void *stream);
+/* The same dL/db for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Nothing is allocated: the partial sums
+ * live in a fixed 4 MiB device array (1048576 floats) that the library holds
+ * on every GPU it runs on until the process ends; call scir_hip_init() on the
+ * device once first (outside capture).  Limit: B * K <= 1048576, else
+ * SCIR_HIP_ERR_INVALID_ARGUMENT with a message.  The result equals
+ * scir_hip_fir_vjp_db_dev's bit for bit whenever that call's B * segments * K
+ * partial sums fit; larger rows use fewer, longer workgroups (the same
+ * deterministic two-stage reduction in another grouping).  Every call on a
+ * GPU shares that GPU's array, which keeps the last call's contents: calls
+ * that overlap on one GPU (on different streams, or a graph replay beside an
+ * eager call) race and must be ordered by the caller. */
+SCIR_HIP_API int scir_hip_fir_vjp_db_noalloc_dev(const float *g, int64_t ldg, const float *x,
+                                                 int64_t ldx, int32_t K, int64_t B, int64_t N,
+                                                 float *db, void *stream);
 
 /* Host tier of both adjoints (the Rust vjp API's entry): host arrays in and
  * out, synchronous, like scir_hip_fir_f32_host.  dx [B, lddx >= N] and db[K]

@@ -57,6 +57,7 @@ SIGNATURES = {
     "scir_hip_fir_ragged_f32_dev": (_I, [_P, _Pi64, _P, _Pi64, _I64, _P, _I32, _P]),
     "scir_hip_fir_vjp_dx_dev": (_I, [_P, _I64, _P, _I32, _I64, _I64, _P, _I64, _I32, _P]),
     "scir_hip_fir_vjp_db_dev": (_I, [_P, _I64, _P, _I64, _I32, _I64, _I64, _P, _P]),
+    "scir_hip_fir_vjp_db_noalloc_dev": (_I, [_P, _I64, _P, _I64, _I32, _I64, _I64, _P, _P]),
     "scir_hip_graph_begin": (_I, [_P]),
     "scir_hip_graph_end": (_I, [_P, _c.POINTER(_P)]),
     "scir_hip_graph_launch": (_I, [_P, _P]),

@@ -57,6 +57,10 @@ struct DeviceState {
     // staging buffers for the chunked copies.
     DeviceBuffer host_x, host_y, host_b;
     DeviceBuffer host_z;  // third matrix of the host-tier adjoint (dL/dx)
+    // Device address of the allocation-free dL/db scratch array (looked up
+    // once when the state is made; null, with the reason, if that failed).
+    float *vjp_noalloc_scratch = nullptr;
+    std::string vjp_noalloc_error;
     void *pinned[4] = {nullptr, nullptr, nullptr, nullptr};
     size_t pinned_bytes = 0;
     hipStream_t host_stream = nullptr;
@@ -108,6 +112,11 @@ int fir_fft_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, i
 int fir_mfma_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
                  float *y, int64_t ldy, int precision, bool rev, hipStream_t stream,
                  DeviceState *st);
+// Partial sums (floats) the allocation-free dL/db entry point holds: 4 MiB.
+constexpr size_t kVjpNoAllocPartials = (size_t)1 << 20;
+// Device address of that entry point's scratch array on the current device.
+int vjp_noalloc_scratch_address(void **ptr);
+
 // Whether the MFMA kernel's LDS holds K taps (one tile per wave).
 bool fir_mfma_takes(int precision, int K);
 int fir_dense_dispatch(const float *x, int64_t ldx, const float *b, int32_t K, int64_t B,

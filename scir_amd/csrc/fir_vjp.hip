@@ -275,7 +275,25 @@ int aligned16(const float *p, int64_t ld)
     return ((reinterpret_cast<uintptr_t>(p) & 15) == 0) && (ld % 4 == 0);
 }
 
+// Partial sums of the allocation-free dL/db entry point: a file-scope device
+// array, one copy per GPU, held until the process ends.  Host code never
+// passes `scratch` itself to a kernel (its name is a host-side stand-in);
+// the device address comes from hipGetSymbolAddress, once per device.
+__device__ __align__(256) float scratch[kVjpNoAllocPartials];
+
 }  // namespace
+
+int vjp_noalloc_scratch_address(void **ptr)
+{
+    *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(SCIR_HIP_ERR_BACKEND, "hipGetSymbolAddress(dL/db scratch) failed: %s",
+                    hipGetErrorString(e));
+    }
+    return SCIR_HIP_OK;
+}
 
 int vjp_db_args(const float *g, int64_t ldg, const float *x, int64_t ldx, int32_t K, int64_t B,
                 int64_t N, const float *db)
@@ -291,9 +309,14 @@ int vjp_db_args(const float *g, int64_t ldg, const float *x, int64_t ldx, int32_
     return SCIR_HIP_OK;
 }
 
-// dL/db on stream s (caller holds st->mu; arguments checked).
+// dL/db on stream s (caller holds st->mu; arguments checked).  With `fixed`,
+// the partial sums go to that device buffer of `fixed_floats` floats instead
+// of the stream's library scratch (nothing is allocated): the FP32 MFMA
+// stage 1 always runs, and when the usual workgroup count would not fit,
+// fewer workgroups per row walk longer runs of tiles.
 int vjp_db_run(const float *g, int64_t ldg, const float *x, int64_t ldx, int32_t K, int64_t B,
-               int64_t N, float *db, hipStream_t s, DeviceState *st)
+               int64_t N, float *db, hipStream_t s, DeviceState *st, float *fixed = nullptr,
+               size_t fixed_floats = 0)
 {
     if (B == 0 || N == 0) {
         // Empty sum: db = 0 (a one-row, zero-work pass of the final kernel).
@@ -307,15 +330,28 @@ int vjp_db_run(const float *g, int64_t ldg, const float *x, int64_t ldx, int32_t
     // contiguous run of tiles of one row.
     const int64_t want = (int64_t)st->cu_count * 8;
     int64_t segs = std::max<int64_t>(1, std::min<int64_t>(tiles_in_row, (want + B - 1) / B));
+    if (fixed) {
+        // B rows of K partial sums at least (one workgroup per row)
+        const int64_t rows_fit = (int64_t)(fixed_floats / (size_t)K);
+        if (B > rows_fit)
+            return fail(SCIR_HIP_ERR_INVALID_ARGUMENT,
+                        "the allocation-free dL/db holds %zu partial sums: rows * K must not "
+                        "exceed it (rows=%lld, K=%d)",
+                        fixed_floats, (long long)B, K);
+        segs = std::max<int64_t>(1, std::min<int64_t>(segs, rows_fit / B));
+    }
     const int64_t tiles_per_wg = (tiles_in_row + segs - 1) / segs;
     segs = (tiles_in_row + tiles_per_wg - 1) / tiles_per_wg;
     const int64_t nwg = B * segs;
     if (nwg > 0x7fffffff) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "batch too large");
     const TuneChoice tc = tune_lookup(K, N);
-    if (tc.vjp_db_mfma) {
-        StreamScratch &sc = scratch_for(st, s);
-        SCIR_TRY(ensure_scratch(st, sc.partials, (size_t)nwg * K * sizeof(float), s));
-        float *partial = static_cast<float *>(sc.partials.ptr);
+    if (tc.vjp_db_mfma || fixed) {
+        float *partial = fixed;
+        if (!fixed) {
+            StreamScratch &sc = scratch_for(st, s);
+            SCIR_TRY(ensure_scratch(st, sc.partials, (size_t)nwg * K * sizeof(float), s));
+            partial = static_cast<float *>(sc.partials.ptr);
+        }
         hipLaunchKernelGGL(fir_vjp_db_mfma_kernel, dim3((unsigned)nwg), dim3(kThreads), 0, s, g,
                            ldg, x, ldx, (int)K, N, (int)segs, (int)tiles_per_wg, partial,
                            aligned16(g, ldg), aligned16(x, ldx));
@@ -376,6 +412,24 @@ int scir_hip_fir_vjp_db_dev(const float *g, int64_t ldg, const float *x, int64_t
         SCIR_TRY(current_device_state(&st));
         std::lock_guard<std::mutex> lk(st->mu);
         return vjp_db_run(g, ldg, x, ldx, K, B, N, db, as_stream(stream), st);
+    });
+}
+
+int scir_hip_fir_vjp_db_noalloc_dev(const float *g, int64_t ldg, const float *x, int64_t ldx,
+                                    int32_t K, int64_t B, int64_t N, float *db, void *stream)
+{
+    return guarded(__func__, [&]() -> int {
+        clear_error();
+        SCIR_TRY(vjp_db_args(g, ldg, x, ldx, K, B, N, db));
+        if (K == 0) return SCIR_HIP_OK;
+        DeviceState *st = nullptr;
+        SCIR_TRY(current_device_state(&st));
+        if (!st->vjp_noalloc_scratch)
+            return fail(SCIR_HIP_ERR_BACKEND, "the dL/db scratch array has no device address: %s",
+                        st->vjp_noalloc_error.c_str());
+        std::lock_guard<std::mutex> lk(st->mu);
+        return vjp_db_run(g, ldg, x, ldx, K, B, N, db, as_stream(stream), st,
+                          st->vjp_noalloc_scratch, kVjpNoAllocPartials);
     });
 }
 

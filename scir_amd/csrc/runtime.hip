@@ -34,6 +34,15 @@ static int init_state(DeviceState *st, int dev)
     st->device = dev;
     st->cu_count = prop.multiProcessorCount;
     st->lds_per_block = prop.sharedMemPerBlock;
+    // The allocation-free dL/db scratch: its address on this device (current
+    // here), kept in the state; every later call uses this pointer.
+    void *scratch = nullptr;
+    if (vjp_noalloc_scratch_address(&scratch) == SCIR_HIP_OK) {
+        st->vjp_noalloc_scratch = static_cast<float *>(scratch);
+    } else {
+        st->vjp_noalloc_error = scir_hip_last_error();
+        clear_error();
+    }
     st->ready = true;
     return SCIR_HIP_OK;
 }
