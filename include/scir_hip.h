@@ -124,7 +124,10 @@ SCIR_HIP_API const char *scir_hip_last_error(void);
 /* Host tier: drop-in for cuda::fir1d_batched_f32_cuda
  * (crates/scir-gpu/src/lib.rs:1036-1113).  x[B, ldx] and y[B, ldy] in host
  * memory, b[K] host; copies in, filters, copies out, synchronises.  B or N == 0
- * is a no-op; K == 0 writes zeros (the CPU loop's empty sum). */
+ * is a no-op; K == 0 writes zeros (the CPU loop's empty sum).  Large batches
+ * run in chunks of rows (tuning key host.chunk_kib) with the copy in of one
+ * chunk, the kernels of the next and the copy out of another overlapping on
+ * three streams; the result does not depend on the chunking. */
 SCIR_HIP_API int scir_hip_fir_f32_host(const float *x, int64_t ldx, const float *b, int32_t K,
                           int64_t B, int64_t N, float *y, int64_t ldy,
                           int32_t precision);
@@ -360,6 +363,9 @@ SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
  *   "auto.fft_min_k"  smallest K for which AUTO (with auto.fft) picks FFT32;
  *                     the autotuner measures it (the K from which FFT32 beats
  *                     BF16x3 MFMA on long rows)
+ *   "host.chunk_kib"  KiB of samples per chunk of the pipelined host tier
+ *                     (default 65536): batches of three chunks or more move
+ *                     in, through the kernels and out in overlapping chunks
  *   "ragged.direct_wg_per_cu"      persistent workgroups per CU of ragged
  *                                  batches in the FP32 direct form (1 .. 32)
  *   "ragged.direct_min_item_kmacs" multiply-adds per direct-form work item,

@@ -48,6 +48,7 @@ int g_over_direct_r = -1, g_over_direct_loader = -1, g_over_direct_seg = -1,
     g_over_dma_bufs = -1;
 int g_over_rg_direct_wg = -1, g_over_rg_direct_kmacs = -1, g_over_rg_mfma_segs = -1,
     g_over_rg_mfma_wg = -1;
+int g_over_host_chunk_kib = -1;
 
 // Buckets searched by the autotuner.
 const int kKBuckets[][2] = {{1, 16}, {17, 64}, {65, 256}, {257, 1017}, {1018, 4096}};
@@ -83,6 +84,12 @@ TuneChoice tune_lookup(int K, int64_t N)
         c.mfma_segs_forced = true;
     }
     return c;
+}
+
+size_t host_chunk_bytes()
+{
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    return g_over_host_chunk_kib > 0 ? (size_t)g_over_host_chunk_kib << 10 : (size_t)64 << 20;
 }
 
 RaggedTune ragged_tune_lookup()
@@ -180,6 +187,9 @@ extern "C" int scir_hip_tune_set(const char *key, int32_t value)
             if (v == 0 || v > 1024)
                 return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "ragged.mfma_segs_per_item: 1 .. 1024");
             g_over_rg_mfma_segs = v;
+        } else if (!std::strcmp(key, "host.chunk_kib")) {
+            if (v == 0) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "host.chunk_kib must be >= 1");
+            g_over_host_chunk_kib = v;
         } else if (!std::strcmp(key, "ragged.mfma_wg_per_cu")) {
             if (v > 8)
                 return fail(SCIR_HIP_ERR_INVALID_ARGUMENT,

@@ -37,6 +37,11 @@
 #include "direct_tiles.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <string>
+#include <thread>
+#include <vector>
 
 namespace scir {
 namespace {
@@ -395,6 +400,160 @@ int fir_dense_dispatch(const float *x, int64_t ldx, const float *b, int32_t K, i
     }
 }
 
+namespace {
+
+// Host-tier copy of `rows` rows between a host array (pitch ldh) and a device
+// array (pitch ldd), on stream s: one linear copy when both are packed.
+hipError_t copy_rows(void *dst, int64_t ldst, const void *src, int64_t lsrc, int64_t N,
+                     int64_t rows, hipMemcpyKind kind, hipStream_t s)
+{
+    const size_t row_bytes = (size_t)N * sizeof(float);
+    if (ldst == N && lsrc == N)
+        return hipMemcpyAsync(dst, src, row_bytes * (size_t)rows, kind, s);
+    return hipMemcpy2DAsync(dst, (size_t)ldst * sizeof(float), src, (size_t)lsrc * sizeof(float),
+                            row_bytes, (size_t)rows, kind, s);
+}
+
+// Shared state of the three host threads of one pipelined host-tier call.
+struct HostPipe {
+    std::mutex m;
+    std::condition_variable cv;
+    int64_t uploaded = 0;  // chunks whose upload event is recorded
+    int64_t computed = 0;  // chunks whose kernel event is recorded
+    std::atomic<bool> failed{false};
+    int status = SCIR_HIP_OK;
+    std::string msg;
+
+    void fail_with(int st, const std::string &what)
+    {
+        std::lock_guard<std::mutex> g(m);
+        if (!failed) {
+            failed = true;
+            status = st;
+            msg = what;
+        }
+        cv.notify_all();
+    }
+    // Waits until `counter` exceeds c; false when the call failed meanwhile.
+    bool wait_for(const int64_t &counter, int64_t c)
+    {
+        std::unique_lock<std::mutex> g(m);
+        cv.wait(g, [&] { return counter > c || failed; });
+        return !failed;
+    }
+    void advance(int64_t &counter, int64_t v)
+    {
+        std::lock_guard<std::mutex> g(m);
+        counter = v;
+        cv.notify_all();
+    }
+};
+
+std::string hip_msg(const char *what, hipError_t e)
+{
+    return std::string(what) + " failed: " + hipGetErrorString(e);
+}
+
+}  // namespace
+
+// The host tier in row chunks so the two PCIe directions and the kernels
+// overlap: an upload thread copies chunk c+1 in while the kernels of chunk c
+// run (enqueued by the calling thread after the chunk's upload event) and a
+// download thread copies finished chunks out (after their kernel events).
+// Pageable host memory makes each copy call return only when its bytes have
+// moved, which is why each direction has a thread of its own.  Device
+// buffers are the full-size host-tier buffers, so no chunk waits for another.
+int host_fir_pipelined(const float *x, int64_t ldx, const float *db, int32_t K, int64_t B,
+                       int64_t N, float *y, int64_t ldy, int32_t precision, float *dx, float *dy,
+                       int64_t ldd, int64_t rows_per_chunk, DeviceState *st)
+{
+    for (hipStream_t &cs : st->host_copy_stream)
+        if (!cs) SCIR_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipStream_t sk = st->host_stream, sup = st->host_copy_stream[0],
+                sdown = st->host_copy_stream[1];
+    const int64_t nch = (B + rows_per_chunk - 1) / rows_per_chunk;
+    host_alloc_point();
+    std::vector<hipEvent_t> ev(2 * (size_t)nch, nullptr);  // [0, nch): uploads, then kernels
+    struct EventsGuard {
+        std::vector<hipEvent_t> &v;
+        ~EventsGuard()
+        {
+            for (hipEvent_t e : v)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } guard{ev};
+    for (hipEvent_t &e : ev) SCIR_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HostPipe pipe;
+    const int dev = st->device;
+    auto rows_of = [&](int64_t c) { return std::min(rows_per_chunk, B - c * rows_per_chunk); };
+    auto up = [&]() {
+        (void)hipSetDevice(dev);  // the current device is per thread
+        for (int64_t c = 0; c < nch; ++c) {
+            if (pipe.failed) return;
+            const int64_t r0 = c * rows_per_chunk;
+            hipError_t e = copy_rows(dx + r0 * ldd, ldd, x + r0 * ldx, ldx, N, rows_of(c),
+                                     hipMemcpyHostToDevice, sup);
+            if (e == hipSuccess) e = hipEventRecord(ev[c], sup);
+            if (e != hipSuccess) return pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("upload", e));
+            pipe.advance(pipe.uploaded, c + 1);
+        }
+    };
+    auto down = [&]() {
+        (void)hipSetDevice(dev);
+        for (int64_t c = 0; c < nch; ++c) {
+            if (!pipe.wait_for(pipe.computed, c)) return;
+            const int64_t r0 = c * rows_per_chunk;
+            hipError_t e = hipStreamWaitEvent(sdown, ev[nch + c], 0);
+            if (e == hipSuccess)
+                e = copy_rows(y + r0 * ldy, ldy, dy + r0 * ldd, ldd, N, rows_of(c),
+                              hipMemcpyDeviceToHost, sdown);
+            if (e != hipSuccess)
+                return pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("download", e));
+        }
+        const hipError_t e = hipStreamSynchronize(sdown);
+        if (e != hipSuccess) pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("download", e));
+    };
+    std::thread tup, tdown;
+    try {
+        tup = std::thread(up);
+        tdown = std::thread(down);
+    } catch (...) {
+        pipe.fail_with(SCIR_HIP_ERR_BACKEND, "host-tier thread start failed");
+        if (tup.joinable()) tup.join();
+        throw;
+    }
+    // Kernels of chunk c once its upload is enqueued (the stream waits for it).
+    for (int64_t c = 0; c < nch; ++c) {
+        if (!pipe.wait_for(pipe.uploaded, c)) break;
+        const int64_t r0 = c * rows_per_chunk;
+        hipError_t e = hipStreamWaitEvent(sk, ev[c], 0);
+        if (e != hipSuccess) {
+            pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("hipStreamWaitEvent", e));
+            break;
+        }
+        const int st2 = fir_dense_dispatch(dx + r0 * ldd, ldd, db, K, rows_of(c), N,
+                                           dy + r0 * ldd, ldd, precision, false, sk, st);
+        if (st2 != SCIR_HIP_OK) {
+            pipe.fail_with(st2, scir_hip_last_error());
+            break;
+        }
+        e = hipEventRecord(ev[nch + c], sk);
+        if (e != hipSuccess) {
+            pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("hipEventRecord", e));
+            break;
+        }
+        pipe.advance(pipe.computed, c + 1);
+    }
+    tup.join();
+    tdown.join();
+    // Leave no copy or kernel of this call running on any of the streams.
+    const hipError_t e1 = hipStreamSynchronize(sup), e2 = hipStreamSynchronize(sk);
+    if (pipe.failed) return fail(pipe.status, "%s", pipe.msg.c_str());
+    if (e1 != hipSuccess) return fail(SCIR_HIP_ERR_BACKEND, "%s", hip_msg("upload", e1).c_str());
+    if (e2 != hipSuccess) return fail(SCIR_HIP_ERR_BACKEND, "%s", hip_msg("kernels", e2).c_str());
+    return SCIR_HIP_OK;
+}
+
 }  // namespace scir
 
 using namespace scir;
@@ -441,6 +600,15 @@ int scir_hip_fir_f32_host(const float *x, int64_t ldx, const float *b, int32_t K
         float *dx = static_cast<float *>(st->host_x.ptr);
         float *dy = static_cast<float *>(st->host_y.ptr);
         float *db = static_cast<float *>(st->host_b.ptr);
+        // Large batches: chunks of rows through the three-stream pipeline.
+        const int64_t rows_per_chunk =
+            std::max<int64_t>(1, (int64_t)(host_chunk_bytes() / ((size_t)ldd * sizeof(float))));
+        if (B >= 3 * rows_per_chunk) {
+            if (K > 0)
+                SCIR_HIP_TRY(hipMemcpy(db, b, (size_t)K * sizeof(float), hipMemcpyHostToDevice));
+            return host_fir_pipelined(x, ldx, db, K, B, N, y, ldy, precision, dx, dy, ldd,
+                                      rows_per_chunk, st);
+        }
         // One linear copy when host and device rows are both packed (a pitched
         // copy of many rows runs as many small transfers).
         if (ldx == N && ldd == N)

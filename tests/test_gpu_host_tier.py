@@ -1,0 +1,68 @@
+"""The drop-in host tier (scir_hip_fir_f32_host) in its chunked form: batches
+of three chunks or more overlap the copy in, the kernels and the copy out on
+three streams.  With the chunk size forced small (tuning key host.chunk_kib),
+every precision gives the same bits as the same call in one piece, f32_exact
+equals the reference loop, and row pitches, a partial last chunk and rows
+longer than a chunk are covered."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import rng
+
+pytestmark = pytest.mark.gpu
+
+
+def host_call(hip, x, ldx, b, B, N, ldy, precision):
+    y = np.full((B, ldy), 7.0, np.float32)
+    hip.check(hip.lib().scir_hip_fir_f32_host(
+        x.ctypes.data_as(ctypes.c_void_p), ldx, b.ctypes.data_as(ctypes.c_void_p), b.size, B, N,
+        y.ctypes.data_as(ctypes.c_void_p), ldy, hip.precision_code(precision)))
+    return y
+
+
+@pytest.fixture
+def chunk_kib(hip):
+    L = hip.lib()
+
+    def set_kib(v):
+        hip.check(L.scir_hip_tune_set(b"host.chunk_kib", v))
+    yield set_kib
+    set_kib(-1)
+
+
+@pytest.mark.parametrize("precision", ["f32_exact", "f32", "bf16x3", "fp16", "fft32", "auto"])
+@pytest.mark.parametrize("N,padx,pady", [(1000, 0, 0), (1001, 3, 1), (4096, 0, 4), (70001, 5, 0)])
+def test_chunked_host_tier_equals_one_piece(hip, oracle, chunk_kib, precision, N, padx, pady):
+    B, K = 37, 129
+    print(f"chunked host tier {precision} N={N} pads=({padx},{pady})", flush=True)
+    r = rng(N + padx + len(precision))
+    ldx, ldy = N + padx, N + pady
+    x = r.standard_normal((B, ldx)).astype(np.float32)
+    b = (r.standard_normal(K) / np.sqrt(K)).astype(np.float32)
+    chunk_kib(1 << 30)  # one piece
+    y1 = host_call(hip, x, ldx, b, B, N, ldy, precision)
+    chunk_kib(16)       # 4096 samples a chunk: 4 rows of 1000, or one row per chunk
+    y2 = host_call(hip, x, ldx, b, B, N, ldy, precision)
+    assert np.array_equal(y1[:, :N], y2[:, :N]), f"{precision}: chunked result differs"
+    assert np.all(y2[:, N:] == 7.0), "padding of y written"
+    if precision == "f32_exact":
+        ref = oracle.fir_ref_f32(np.ascontiguousarray(x[:, :N]), b[::-1].copy())
+        assert np.array_equal(y2[:, :N], ref)
+
+
+def test_chunked_host_tier_reports_bad_precision_and_recovers(hip, chunk_kib):
+    """An error inside the pipeline (the kernel dispatch refuses the
+    precision) comes back as a status, and the next call works."""
+    B, N, K = 40, 2000, 7000  # K beyond the BF16x3 kernel's LDS: SCIR_HIP_ERR_UNSUPPORTED
+    x = rng(3).standard_normal((B, N)).astype(np.float32)
+    b = (rng(4).standard_normal(K) / 100).astype(np.float32)
+    chunk_kib(16)
+    with pytest.raises(hip.HipError) as e:
+        host_call(hip, x, N, b, B, N, N, "bf16x3")
+    assert e.value.status == hip.ERR_UNSUPPORTED
+    y = host_call(hip, x, N, b[:33], B, N, N, "f32_exact")
+    assert np.isfinite(y).all()
