@@ -124,10 +124,12 @@ SCIR_HIP_API const char *scir_hip_last_error(void);
 /* Host tier: drop-in for cuda::fir1d_batched_f32_cuda
  * (crates/scir-gpu/src/lib.rs:1036-1113).  x[B, ldx] and y[B, ldy] in host
  * memory, b[K] host; copies in, filters, copies out, synchronises.  B or N == 0
- * is a no-op; K == 0 writes zeros (the CPU loop's empty sum).  Large batches
- * run in chunks of rows (tuning key host.chunk_kib) with the copy in of one
- * chunk, the kernels of the next and the copy out of another overlapping on
- * three streams; the result does not depend on the chunking. */
+ * is a no-op; K == 0 writes zeros (the CPU loop's empty sum).  Batches of
+ * three chunks of rows or more (tuning key host.chunk_kib) are staged through
+ * pinned host memory by host.copy_threads threads per direction, with the
+ * copy in of one chunk, the kernels of another and the copy out of a third
+ * overlapping on three streams; the result does not depend on the chunking.
+ * The library keeps four pinned blocks of one chunk each until shutdown. */
 SCIR_HIP_API int scir_hip_fir_f32_host(const float *x, int64_t ldx, const float *b, int32_t K,
                           int64_t B, int64_t N, float *y, int64_t ldy,
                           int32_t precision);
@@ -364,8 +366,11 @@ SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
  *                     the autotuner measures it (the K from which FFT32 beats
  *                     BF16x3 MFMA on long rows)
  *   "host.chunk_kib"  KiB of samples per chunk of the pipelined host tier
- *                     (default 65536): batches of three chunks or more move
+ *                     (default 16384): batches of three chunks or more move
  *                     in, through the kernels and out in overlapping chunks
+ *   "host.copy_threads" threads per direction that copy the host tier's
+ *                     chunks between the caller's arrays and pinned staging
+ *                     memory (1 .. 64, default 4)
  *   "ragged.direct_wg_per_cu"      persistent workgroups per CU of ragged
  *                                  batches in the FP32 direct form (1 .. 32)
  *   "ragged.direct_min_item_kmacs" multiply-adds per direct-form work item,

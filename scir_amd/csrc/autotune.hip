@@ -49,6 +49,7 @@ int g_over_direct_r = -1, g_over_direct_loader = -1, g_over_direct_seg = -1,
 int g_over_rg_direct_wg = -1, g_over_rg_direct_kmacs = -1, g_over_rg_mfma_segs = -1,
     g_over_rg_mfma_wg = -1;
 int g_over_host_chunk_kib = -1;
+int g_over_host_copy_threads = -1;
 
 // Buckets searched by the autotuner.
 const int kKBuckets[][2] = {{1, 16}, {17, 64}, {65, 256}, {257, 1017}, {1018, 4096}};
@@ -89,7 +90,13 @@ TuneChoice tune_lookup(int K, int64_t N)
 size_t host_chunk_bytes()
 {
     std::lock_guard<std::mutex> g(g_tune_mu);
-    return g_over_host_chunk_kib > 0 ? (size_t)g_over_host_chunk_kib << 10 : (size_t)64 << 20;
+    return g_over_host_chunk_kib > 0 ? (size_t)g_over_host_chunk_kib << 10 : (size_t)16 << 20;
+}
+
+int host_copy_threads()
+{
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    return g_over_host_copy_threads > 0 ? g_over_host_copy_threads : 4;
 }
 
 RaggedTune ragged_tune_lookup()
@@ -190,6 +197,10 @@ extern "C" int scir_hip_tune_set(const char *key, int32_t value)
         } else if (!std::strcmp(key, "host.chunk_kib")) {
             if (v == 0) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "host.chunk_kib must be >= 1");
             g_over_host_chunk_kib = v;
+        } else if (!std::strcmp(key, "host.copy_threads")) {
+            if (v == 0 || v > 64)
+                return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "host.copy_threads: 1 .. 64");
+            g_over_host_copy_threads = v;
         } else if (!std::strcmp(key, "ragged.mfma_wg_per_cu")) {
             if (v > 8)
                 return fail(SCIR_HIP_ERR_INVALID_ARGUMENT,

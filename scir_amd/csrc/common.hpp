@@ -138,7 +138,10 @@ int fir_mfma_ragged_launch(const float *x, const float *b, float *y, const MfmaR
 TuneChoice tune_lookup(int K, int64_t N);
 RaggedTune ragged_tune_lookup();
 // Bytes of samples per chunk of the pipelined host tier (tuning key
-// host.chunk_kib; 64 MiB by default).
+// host.chunk_kib; 16 MiB by default).
 size_t host_chunk_bytes();
+// Threads per direction for the host tier's staging copies through pinned
+// memory (tuning key host.copy_threads; 4 by default).
+int host_copy_threads();
 
 }  // namespace scir

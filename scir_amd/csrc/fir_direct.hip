@@ -39,6 +39,7 @@
 #include <algorithm>
 #include <atomic>
 #include <condition_variable>
+#include <cstring>
 #include <string>
 #include <thread>
 #include <vector>
@@ -402,18 +403,6 @@ int fir_dense_dispatch(const float *x, int64_t ldx, const float *b, int32_t K, i
 
 namespace {
 
-// Host-tier copy of `rows` rows between a host array (pitch ldh) and a device
-// array (pitch ldd), on stream s: one linear copy when both are packed.
-hipError_t copy_rows(void *dst, int64_t ldst, const void *src, int64_t lsrc, int64_t N,
-                     int64_t rows, hipMemcpyKind kind, hipStream_t s)
-{
-    const size_t row_bytes = (size_t)N * sizeof(float);
-    if (ldst == N && lsrc == N)
-        return hipMemcpyAsync(dst, src, row_bytes * (size_t)rows, kind, s);
-    return hipMemcpy2DAsync(dst, (size_t)ldst * sizeof(float), src, (size_t)lsrc * sizeof(float),
-                            row_bytes, (size_t)rows, kind, s);
-}
-
 // Shared state of the three host threads of one pipelined host-tier call.
 struct HostPipe {
     std::mutex m;
@@ -449,6 +438,129 @@ struct HostPipe {
     }
 };
 
+// A team of threads for the staging copies of one direction: run(job) calls
+// job(member, size) on every member, the calling thread being member 0, and
+// returns when all have finished.  The job is passed by address, so a run
+// allocates nothing and cannot throw.
+class CopyTeam {
+public:
+    explicit CopyTeam(int size) : size_(size)
+    {
+        try {
+            for (int i = 1; i < size; ++i) helpers_.emplace_back([this, i] { serve(i); });
+        } catch (...) {
+            stop();
+            throw;
+        }
+    }
+    ~CopyTeam() { stop(); }
+    CopyTeam(const CopyTeam &) = delete;
+    CopyTeam &operator=(const CopyTeam &) = delete;
+
+    template <class F> void run(F &job)
+    {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            fn_ = [](void *ctx, int member, int size) { (*static_cast<F *>(ctx))(member, size); };
+            ctx_ = &job;
+            pending_ = size_ - 1;
+            ++generation_;
+        }
+        go_.notify_all();
+        job(0, size_);
+        std::unique_lock<std::mutex> g(m_);
+        done_.wait(g, [&] { return pending_ == 0; });
+    }
+
+private:
+    void serve(int member)
+    {
+        uint64_t seen = 0;
+        for (;;) {
+            void (*fn)(void *, int, int);
+            void *ctx;
+            {
+                std::unique_lock<std::mutex> g(m_);
+                go_.wait(g, [&] { return stopping_ || generation_ != seen; });
+                if (stopping_) return;
+                seen = generation_;
+                fn = fn_;
+                ctx = ctx_;
+            }
+            fn(ctx, member, size_);
+            std::lock_guard<std::mutex> g(m_);
+            if (--pending_ == 0) done_.notify_all();
+        }
+    }
+    void stop()
+    {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stopping_ = true;
+        }
+        go_.notify_all();
+        for (std::thread &t : helpers_) t.join();
+        helpers_.clear();
+    }
+
+    const int size_;
+    std::vector<std::thread> helpers_;
+    std::mutex m_;
+    std::condition_variable go_, done_;
+    void (*fn_)(void *, int, int) = nullptr;
+    void *ctx_ = nullptr;
+    uint64_t generation_ = 0;
+    int pending_ = 0;
+    bool stopping_ = false;
+};
+
+// Member m of n copies its share of a rows x N block of floats from pitch
+// lsrc to pitch ldst: the block's samples in row-major order, cut into n
+// ranges that start on 64-byte boundaries of that order.
+void copy_share(float *dst, int64_t ldst, const float *src, int64_t lsrc, int64_t N,
+                int64_t rows, int m, int n)
+{
+    const int64_t total = rows * N;
+    int64_t lo = (total / n * m + total % n * m / n) & ~int64_t(15);
+    const int64_t hi =
+        m + 1 == n ? total : (total / n * (m + 1) + total % n * (m + 1) / n) & ~int64_t(15);
+    while (lo < hi) {
+        const int64_t r = lo / N, c = lo - r * N, len = std::min(hi - lo, N - c);
+        std::memcpy(dst + r * ldst + c, src + r * lsrc + c, (size_t)len * sizeof(float));
+        lo += len;
+    }
+}
+
+// Four pinned host blocks of at least `bytes` each (two staging slots per
+// direction), kept in the device state until shutdown.
+int ensure_pinned(DeviceState *st, size_t bytes)
+{
+    if (st->pinned_bytes >= bytes) return SCIR_HIP_OK;
+    for (void *&p : st->pinned) {
+        if (p) SCIR_HIP_TRY(hipHostFree(p));
+        p = nullptr;
+    }
+    st->pinned_bytes = 0;
+    if (alloc_failure_injected())
+        return fail(SCIR_HIP_ERR_OUT_OF_MEMORY, "host-tier staging: pinned allocation of %zu "
+                    "bytes failed (SCIR_HIP_FAIL_ALLOC is set)", bytes);
+    for (void *&p : st->pinned) {
+        const hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            for (void *&q : st->pinned) {
+                if (q) (void)hipHostFree(q);
+                q = nullptr;
+            }
+            return fail(e == hipErrorOutOfMemory ? SCIR_HIP_ERR_OUT_OF_MEMORY : SCIR_HIP_ERR_BACKEND,
+                        "host-tier staging: hipHostMalloc(%zu) failed: %s", bytes,
+                        hipGetErrorString(e));
+        }
+    }
+    st->pinned_bytes = bytes;
+    return SCIR_HIP_OK;
+}
+
 std::string hip_msg(const char *what, hipError_t e)
 {
     return std::string(what) + " failed: " + hipGetErrorString(e);
@@ -456,13 +568,20 @@ std::string hip_msg(const char *what, hipError_t e)
 
 }  // namespace
 
-// The host tier in row chunks so the two PCIe directions and the kernels
-// overlap: an upload thread copies chunk c+1 in while the kernels of chunk c
-// run (enqueued by the calling thread after the chunk's upload event) and a
-// download thread copies finished chunks out (after their kernel events).
-// Pageable host memory makes each copy call return only when its bytes have
-// moved, which is why each direction has a thread of its own.  Device
-// buffers are the full-size host-tier buffers, so no chunk waits for another.
+// The host tier in row chunks staged through pinned memory, so that both
+// PCIe directions run at the DMA engines' rate and overlap each other and the
+// kernels.  Chunk c goes through five stages:
+//   1. the input thread's team copies its rows from x into pinned slot c % 2
+//      (after the upload of chunk c - 2 out of that slot has finished);
+//   2. one copy moves the slot to the device (upload stream, event up[c]);
+//   3. the calling thread enqueues the chunk's kernels behind up[c] (kernel
+//      stream, event k[c]);
+//   4. the output thread copies the chunk's result into pinned slot c % 2 of
+//      the output side (download stream, behind k[c], event down[c]);
+//   5. after down[c], the output thread's team copies the rows into y.
+// The output thread enqueues the download of chunk c + 1 before it waits for
+// chunk c, so the DMA engine and the staging copies overlap in both
+// directions.  Device buffers are the full-size host-tier buffers.
 int host_fir_pipelined(const float *x, int64_t ldx, const float *db, int32_t K, int64_t B,
                        int64_t N, float *y, int64_t ldy, int32_t precision, float *dx, float *dy,
                        int64_t ldd, int64_t rows_per_chunk, DeviceState *st)
@@ -472,8 +591,15 @@ int host_fir_pipelined(const float *x, int64_t ldx, const float *db, int32_t K, 
     hipStream_t sk = st->host_stream, sup = st->host_copy_stream[0],
                 sdown = st->host_copy_stream[1];
     const int64_t nch = (B + rows_per_chunk - 1) / rows_per_chunk;
+    const size_t slot_bytes = (size_t)rows_per_chunk * (size_t)ldd * sizeof(float);
+    SCIR_TRY(ensure_pinned(st, slot_bytes));
+    float *const pin_in[2] = {static_cast<float *>(st->pinned[0]),
+                              static_cast<float *>(st->pinned[1])};
+    float *const pin_out[2] = {static_cast<float *>(st->pinned[2]),
+                               static_cast<float *>(st->pinned[3])};
     host_alloc_point();
-    std::vector<hipEvent_t> ev(2 * (size_t)nch, nullptr);  // [0, nch): uploads, then kernels
+    // [0, nch): uploads; [nch, 2 nch): kernels; [2 nch, 3 nch): downloads.
+    std::vector<hipEvent_t> ev(3 * (size_t)nch, nullptr);
     struct EventsGuard {
         std::vector<hipEvent_t> &v;
         ~EventsGuard()
@@ -483,43 +609,68 @@ int host_fir_pipelined(const float *x, int64_t ldx, const float *db, int32_t K, 
         }
     } guard{ev};
     for (hipEvent_t &e : ev) SCIR_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    const int threads = host_copy_threads();
+    CopyTeam team_in(threads), team_out(threads);
     HostPipe pipe;
     const int dev = st->device;
     auto rows_of = [&](int64_t c) { return std::min(rows_per_chunk, B - c * rows_per_chunk); };
-    auto up = [&]() {
+    auto in = [&]() {
         (void)hipSetDevice(dev);  // the current device is per thread
         for (int64_t c = 0; c < nch; ++c) {
             if (pipe.failed) return;
-            const int64_t r0 = c * rows_per_chunk;
-            hipError_t e = copy_rows(dx + r0 * ldd, ldd, x + r0 * ldx, ldx, N, rows_of(c),
-                                     hipMemcpyHostToDevice, sup);
+            hipError_t e = c >= 2 ? hipEventSynchronize(ev[c - 2]) : hipSuccess;
+            if (e != hipSuccess) return pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("upload", e));
+            const int64_t r0 = c * rows_per_chunk, rows = rows_of(c);
+            float *slot = pin_in[c & 1];
+            const float *src = x + r0 * ldx;
+            auto stage = [&](int m, int n) { copy_share(slot, ldd, src, ldx, N, rows, m, n); };
+            team_in.run(stage);
+            e = hipMemcpyAsync(dx + r0 * ldd, slot, (size_t)rows * ldd * sizeof(float),
+                               hipMemcpyHostToDevice, sup);
             if (e == hipSuccess) e = hipEventRecord(ev[c], sup);
             if (e != hipSuccess) return pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("upload", e));
             pipe.advance(pipe.uploaded, c + 1);
         }
     };
-    auto down = [&]() {
+    auto out = [&]() {
         (void)hipSetDevice(dev);
-        for (int64_t c = 0; c < nch; ++c) {
-            if (!pipe.wait_for(pipe.computed, c)) return;
+        auto enqueue = [&](int64_t c) {
+            if (!pipe.wait_for(pipe.computed, c)) return false;
             const int64_t r0 = c * rows_per_chunk;
             hipError_t e = hipStreamWaitEvent(sdown, ev[nch + c], 0);
             if (e == hipSuccess)
-                e = copy_rows(y + r0 * ldy, ldy, dy + r0 * ldd, ldd, N, rows_of(c),
-                              hipMemcpyDeviceToHost, sdown);
+                e = hipMemcpyAsync(pin_out[c & 1], dy + r0 * ldd,
+                                   (size_t)rows_of(c) * ldd * sizeof(float),
+                                   hipMemcpyDeviceToHost, sdown);
+            if (e == hipSuccess) e = hipEventRecord(ev[2 * nch + c], sdown);
+            if (e != hipSuccess) {
+                pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("download", e));
+                return false;
+            }
+            return true;
+        };
+        if (!enqueue(0)) return;
+        for (int64_t c = 0; c < nch; ++c) {
+            // Slot (c + 1) % 2 last held chunk c - 1, already copied out.
+            if (c + 1 < nch && !enqueue(c + 1)) return;
+            const hipError_t e = hipEventSynchronize(ev[2 * nch + c]);
             if (e != hipSuccess)
                 return pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("download", e));
+            if (pipe.failed) return;
+            const int64_t r0 = c * rows_per_chunk, rows = rows_of(c);
+            const float *slot = pin_out[c & 1];
+            float *dst = y + r0 * ldy;
+            auto unstage = [&](int m, int n) { copy_share(dst, ldy, slot, ldd, N, rows, m, n); };
+            team_out.run(unstage);
         }
-        const hipError_t e = hipStreamSynchronize(sdown);
-        if (e != hipSuccess) pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("download", e));
     };
-    std::thread tup, tdown;
+    std::thread tin, tout;
     try {
-        tup = std::thread(up);
-        tdown = std::thread(down);
+        tin = std::thread(in);
+        tout = std::thread(out);
     } catch (...) {
         pipe.fail_with(SCIR_HIP_ERR_BACKEND, "host-tier thread start failed");
-        if (tup.joinable()) tup.join();
+        if (tin.joinable()) tin.join();
         throw;
     }
     // Kernels of chunk c once its upload is enqueued (the stream waits for it).
@@ -544,13 +695,16 @@ int host_fir_pipelined(const float *x, int64_t ldx, const float *db, int32_t K, 
         }
         pipe.advance(pipe.computed, c + 1);
     }
-    tup.join();
-    tdown.join();
-    // Leave no copy or kernel of this call running on any of the streams.
-    const hipError_t e1 = hipStreamSynchronize(sup), e2 = hipStreamSynchronize(sk);
+    tin.join();
+    tout.join();
+    // Leave no copy or kernel of this call running on any of the streams (the
+    // pinned slots are reused by the next call).
+    const hipError_t e1 = hipStreamSynchronize(sup), e2 = hipStreamSynchronize(sk),
+                     e3 = hipStreamSynchronize(sdown);
     if (pipe.failed) return fail(pipe.status, "%s", pipe.msg.c_str());
     if (e1 != hipSuccess) return fail(SCIR_HIP_ERR_BACKEND, "%s", hip_msg("upload", e1).c_str());
     if (e2 != hipSuccess) return fail(SCIR_HIP_ERR_BACKEND, "%s", hip_msg("kernels", e2).c_str());
+    if (e3 != hipSuccess) return fail(SCIR_HIP_ERR_BACKEND, "%s", hip_msg("download", e3).c_str());
     return SCIR_HIP_OK;
 }
 

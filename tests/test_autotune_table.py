@@ -52,6 +52,12 @@ def test_overrides_validated():
     assert L.scir_hip_tune_set(b"no.such.key", 1) == _lib.ERR_INVALID_ARGUMENT
     assert L.scir_hip_tune_set(b"direct.r", 16) == _lib.OK
     assert L.scir_hip_tune_set(b"direct.r", -1) == _lib.OK
+    assert L.scir_hip_tune_set(b"host.chunk_kib", 0) == _lib.ERR_INVALID_ARGUMENT
+    assert L.scir_hip_tune_set(b"host.copy_threads", 0) == _lib.ERR_INVALID_ARGUMENT
+    assert b"host.copy_threads" in L.scir_hip_last_error()
+    assert L.scir_hip_tune_set(b"host.copy_threads", 65) == _lib.ERR_INVALID_ARGUMENT
+    assert L.scir_hip_tune_set(b"host.copy_threads", 64) == _lib.OK
+    assert L.scir_hip_tune_set(b"host.copy_threads", -1) == _lib.OK
 
 
 RAGGED = """{"arch": "gfx950", "auto_mfma_min_k": 128, "auto_fft_min_k": 384,

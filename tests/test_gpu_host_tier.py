@@ -1,9 +1,11 @@
 """The drop-in host tier (scir_hip_fir_f32_host) in its chunked form: batches
-of three chunks or more overlap the copy in, the kernels and the copy out on
-three streams.  With the chunk size forced small (tuning key host.chunk_kib),
+of three chunks or more are staged through pinned memory by teams of copy
+threads and overlap the copy in, the kernels and the copy out on three
+streams.  With the chunk size forced small (tuning key host.chunk_kib),
 every precision gives the same bits as the same call in one piece, f32_exact
-equals the reference loop, and row pitches, a partial last chunk and rows
-longer than a chunk are covered."""
+equals the reference loop, and row pitches, a partial last chunk, rows
+longer than a chunk and teams of 1 to 8 threads (host.copy_threads, which
+cut a chunk into shares that end inside rows) are covered."""
 from __future__ import annotations
 
 import ctypes
@@ -32,6 +34,16 @@ def chunk_kib(hip):
         hip.check(L.scir_hip_tune_set(b"host.chunk_kib", v))
     yield set_kib
     set_kib(-1)
+
+
+@pytest.fixture
+def copy_threads(hip):
+    L = hip.lib()
+
+    def set_threads(v):
+        hip.check(L.scir_hip_tune_set(b"host.copy_threads", v))
+    yield set_threads
+    set_threads(-1)
 
 
 @pytest.mark.parametrize("precision", ["f32_exact", "f32", "bf16x3", "fp16", "fft32", "auto"])
@@ -66,3 +78,45 @@ def test_chunked_host_tier_reports_bad_precision_and_recovers(hip, chunk_kib):
     assert e.value.status == hip.ERR_UNSUPPORTED
     y = host_call(hip, x, N, b[:33], B, N, N, "f32_exact")
     assert np.isfinite(y).all()
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+@pytest.mark.parametrize("N,padx,pady", [(1000, 0, 0), (999, 1, 5), (20011, 0, 3)])
+def test_copy_team_sizes(hip, oracle, chunk_kib, copy_threads, threads, N, padx, pady):
+    """Shares of a chunk cut at 64-byte points of its row-major order: with 3
+    or 8 members most shares start and end inside rows, and every sample
+    still lands once, in its place, with the pads of y untouched."""
+    B, K = 53, 33
+    print(f"host tier copy team {threads} N={N} pads=({padx},{pady})", flush=True)
+    r = rng(N * 7 + threads)
+    ldx, ldy = N + padx, N + pady
+    x = r.standard_normal((B, ldx)).astype(np.float32)
+    b = (r.standard_normal(K) / np.sqrt(K)).astype(np.float32)
+    chunk_kib(20)  # 5120 samples: 5 rows of 1000, or part of a 20011-sample row
+    copy_threads(threads)
+    y = host_call(hip, x, ldx, b, B, N, ldy, "f32_exact")
+    ref = oracle.fir_ref_f32(np.ascontiguousarray(x[:, :N]), b[::-1].copy())
+    assert np.array_equal(y[:, :N], ref)
+    assert np.all(y[:, N:] == 7.0), "padding of y written"
+
+
+def test_pinned_staging_allocation_failure_reported(hip, oracle, chunk_kib, monkeypatch):
+    """The pinned staging blocks grow on the first chunked call that needs
+    larger ones; with allocation failure injected that call returns
+    ERR_OUT_OF_MEMORY, and the next call (injection off) works.  Rows of 24 MB
+    make the blocks larger than any other test's."""
+    B, N, K = 3, 6_000_000, 17
+    print("host tier pinned allocation failure", flush=True)
+    x = rng(11).standard_normal((B, N)).astype(np.float32)
+    b = (rng(12).standard_normal(K) / 4).astype(np.float32)
+    chunk_kib(1 << 30)
+    host_call(hip, x, N, b, B, N, N, "f32")  # device buffers sized outside injection
+    chunk_kib(1)  # one row a chunk: three chunks, staged through 24 MB slots
+    monkeypatch.setenv("SCIR_HIP_FAIL_ALLOC", "1")
+    with pytest.raises(hip.HipError) as e:
+        host_call(hip, x, N, b, B, N, N, "f32")
+    assert e.value.status == hip.ERR_OUT_OF_MEMORY
+    monkeypatch.delenv("SCIR_HIP_FAIL_ALLOC")
+    y = host_call(hip, x, N, b, B, N, N, "f32_exact")
+    ref = oracle.fir_ref_f32(x, b[::-1].copy())
+    assert np.array_equal(y, ref)
