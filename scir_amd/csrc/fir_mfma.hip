@@ -57,6 +57,11 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+// Four floats at any 4-byte aligned global address: the kernels load rows and
+// store outputs with 16-byte accesses whatever the row's offset or pitch (the
+// hardware runs in unaligned access mode; an access that crosses a cache line
+// costs two requests, far less than four 4-byte accesses).
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;
@@ -262,20 +267,20 @@ struct Contraction {
 // zero outside [0, N) and for u >= units).  In rev mode logical sample i is
 // physical sample N-1-i.
 __device__ __forceinline__ void load_unit(const float *__restrict__ xrow, int64_t N, int64_t xb,
-                                          int u, int units, int vec_x, int rev, f32x4 (&r)[2])
+                                          int u, int units, int rev, f32x4 (&r)[2])
 {
     const int64_t g0 = xb + 8 * (int64_t)u;
     if (u >= units) {
         r[0] = r[1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    } else if (vec_x && g0 >= 0 && g0 + 7 < N) {
+    } else if (g0 >= 0 && g0 + 7 < N) {
         if (!rev) {
-            r[0] = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0));
-            r[1] = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g0 + 4));
+            r[0] = __builtin_nontemporal_load(reinterpret_cast<const f32x4u *>(xrow + g0));
+            r[1] = __builtin_nontemporal_load(reinterpret_cast<const f32x4u *>(xrow + g0 + 4));
         } else {  // logical g0..g0+7 == physical N-1-g0 .. N-8-g0
             const f32x4 a =
-                __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + (N - 4 - g0)));
+                __builtin_nontemporal_load(reinterpret_cast<const f32x4u *>(xrow + (N - 4 - g0)));
             const f32x4 c =
-                __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + (N - 8 - g0)));
+                __builtin_nontemporal_load(reinterpret_cast<const f32x4u *>(xrow + (N - 8 - g0)));
             r[0] = f32x4{a.w, a.z, a.y, a.x};
             r[1] = f32x4{c.w, c.z, c.y, c.x};
         }
@@ -343,7 +348,7 @@ template <int S, int TW, int PF>
 __device__ __forceinline__ void stage_segment(unsigned char *lds, int split_stride,
                                               const f32x4 (&raw)[PF][2],
                                               const float *__restrict__ xrow, int64_t N,
-                                              int64_t xb, int units, int vec_x, int rev, int tid)
+                                              int64_t xb, int units, int rev, int tid)
 {
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
@@ -354,7 +359,7 @@ __device__ __forceinline__ void stage_segment(unsigned char *lds, int split_stri
         f32x4 rest[kRestBatch][2];
 #pragma unroll
         for (int j = 0; j < kRestBatch; ++j)
-            load_unit(xrow, N, xb, u0 + j * kThreads, units, vec_x, rev, rest[j]);
+            load_unit(xrow, N, xb, u0 + j * kThreads, units, rev, rest[j]);
 #pragma unroll
         for (int j = 0; j < kRestBatch; ++j) {
             const int u = u0 + j * kThreads;
@@ -398,7 +403,7 @@ __device__ __forceinline__ void contract_segment(Contraction<S, TW> &ct, const u
 template <int S, int TW>
 __device__ __forceinline__ void store_segment(const Contraction<S, TW> &ct,
                                               float *__restrict__ yrow, int64_t N, int64_t P0,
-                                              int wave, int i, int g, int vec_y, int rev,
+                                              int wave, int i, int g, int rev,
                                               const float *__restrict__ xrow,
                                               const float *__restrict__ b, int K)
 {
@@ -420,11 +425,11 @@ __device__ __forceinline__ void store_segment(const Contraction<S, TW> &ct,
                     if (n + j < N && !__builtin_isfinite(v[j]))
                         v[j] = direct_output<false>(xrow, N, rev != 0, b, K, n + j);
             }
-            if (vec_y && n + 3 < N) {
+            if (n + 3 < N) {
                 if (!rev)
-                    *reinterpret_cast<f32x4 *>(yrow + n) = v;
+                    *reinterpret_cast<f32x4u *>(yrow + n) = v;
                 else
-                    *reinterpret_cast<f32x4 *>(yrow + (N - 4 - n)) = f32x4{v.w, v.z, v.y, v.x};
+                    *reinterpret_cast<f32x4u *>(yrow + (N - 4 - n)) = f32x4{v.w, v.z, v.y, v.x};
             } else {
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
@@ -447,7 +452,7 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
     const float *__restrict__ x, int64_t ldx, int64_t N, float *__restrict__ y, int64_t ldy,
     const float *__restrict__ b, int K, int msteps, int m_hi, int64_t row0, int segs_per_row,
     int64_t nsegs, int segs_per_wg, int64_t nwg, int split_stride, int tab_offset,
-    int copy_bytes, int vec_x, int vec_y, int rev)
+    int copy_bytes, int rev)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     constexpr int SEG_BLOCKS = kWaves * TW * 32;
@@ -472,7 +477,7 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
         seg_base(sg, xrow, xb, P0, row);
 #pragma unroll
         for (int j = 0; j < PF; ++j)
-            load_unit(xrow, N, xb, tid + j * kThreads, units, vec_x, rev, raw[j]);
+            load_unit(xrow, N, xb, tid + j * kThreads, units, rev, raw[j]);
     };
     prefetch(seg_first, (int)threadIdx.x);
 
@@ -493,7 +498,7 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
 
         // ---- stage: x[xb .. xb + 8*units) -> S split arrays in LDS ----
         if (sg != seg_first) __syncthreads();  // the previous segment's MFMAs are done
-        stage_segment<S, TW, PF>(lds, split_stride, raw, xrow, N, xb, units, vec_x, rev, tid);
+        stage_segment<S, TW, PF>(lds, split_stride, raw, xrow, N, xb, units, rev, tid);
         __syncthreads();
         if (sg + 1 < seg_end) prefetch(sg + 1, tid);
 
@@ -501,7 +506,7 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
         Contraction<S, TW> ct;
         contract_segment<S, TW>(ct, lds, split_stride, tab_offset, copy_bytes, msteps, m_hi, wave,
                                 i, g);
-        store_segment<S, TW>(ct, y + row * ldy, N, P0, wave, i, g, vec_y, rev, xrow, b, K);
+        store_segment<S, TW>(ct, y + row * ldy, N, P0, wave, i, g, rev, xrow, b, K);
     }
 }
 
@@ -536,14 +541,12 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_ragged_kernel(
         const int units = SEG_BLOCKS * 4 + 2 * m_hi;
         const float *__restrict__ xrow = x + w.x0;
         float *__restrict__ yrow = y + w.x0;
-        const int vec_x = (reinterpret_cast<uintptr_t>(xrow) & 15) == 0;
-        const int vec_y = (reinterpret_cast<uintptr_t>(yrow) & 15) == 0;
         const int64_t seg_end = (int64_t)w.seg0 + w.nseg;
         auto prefetch = [&](int64_t sg, int tid) {
             const int64_t xb = 32 * sg * SEG_BLOCKS - 16 * (int64_t)m_hi;
 #pragma unroll
             for (int j = 0; j < PF; ++j)
-                load_unit(xrow, w.len, xb, tid + j * kThreads, units, vec_x, 0, raw[j]);
+                load_unit(xrow, w.len, xb, tid + j * kThreads, units, 0, raw[j]);
         };
         prefetch(w.seg0, (int)threadIdx.x);
         if (w.b0 != tab_b0 || w.K != tab_K) {
@@ -559,14 +562,13 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_ragged_kernel(
             const int64_t P0 = sg * SEG_BLOCKS;
             const int64_t xb = 32 * P0 - 16 * (int64_t)m_hi;
             if (sg != w.seg0) __syncthreads();  // the previous segment's MFMAs are done
-            stage_segment<S, TW, PF>(lds, split_stride, raw, xrow, w.len, xb, units, vec_x, 0,
-                                     tid);
+            stage_segment<S, TW, PF>(lds, split_stride, raw, xrow, w.len, xb, units, 0, tid);
             __syncthreads();
             if (sg + 1 < seg_end) prefetch(sg + 1, tid);
             Contraction<S, TW> ct;
             contract_segment<S, TW>(ct, lds, split_stride, tab_offset, copy_bytes, msteps, m_hi,
                                     wave, i, g);
-            store_segment<S, TW>(ct, yrow, w.len, P0, wave, i, g, vec_y, 0, xrow, b + w.b0,
+            store_segment<S, TW>(ct, yrow, w.len, P0, wave, i, g, 0, xrow, b + w.b0,
                                  w.K);
         }
     }
@@ -574,11 +576,6 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_ragged_kernel(
 
 constexpr int kRaggedTW = 2;
 static_assert(kMfmaRaggedSegLen == 32 * kWaves * kRaggedTW * 32, "segment length");
-
-int align_ok(const float *p, int64_t ld)
-{
-    return ((reinterpret_cast<uintptr_t>(p) & 15) == 0) && (ld % 4 == 0);
-}
 
 template <int S, int TW>
 size_t mfma_lds_bytes(int K)
@@ -605,8 +602,6 @@ int launch_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, i
     const int64_t segs = (N + seg_len - 1) / seg_len;
     if (segs > 0x7fffffff)
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row too long (N=%lld)", (long long)N);
-    const int nvec = !rev || (N % 4 == 0);
-    const int vec_x = align_ok(x, ldx) && nvec, vec_y = align_ok(y, ldy) && nvec;
     const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / segs);
     for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
         const int64_t rows = std::min(rows_per_launch, B - r0);
@@ -614,8 +609,7 @@ int launch_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, i
         const int64_t nwg = (nsegs + segs_per_wg - 1) / segs_per_wg;
         hipLaunchKernelGGL((fir_mfma_kernel<S, TW>), dim3((unsigned)nwg), dim3(kThreads), lds,
                            stream, x, ldx, N, y, ldy, b, K, msteps, m_hi, r0, (int)segs, nsegs,
-                           segs_per_wg, nwg, split_stride, tab_offset, copy_bytes, vec_x, vec_y,
-                           (int)rev);
+                           segs_per_wg, nwg, split_stride, tab_offset, copy_bytes, (int)rev);
         SCIR_TRY(check_launch("fir_mfma_kernel"));
     }
     return SCIR_HIP_OK;
