@@ -317,27 +317,41 @@ __device__ __forceinline__ void store_unit(unsigned char *lds, int split_stride,
 
 // Tap tables for taps b[0, K) (lfilter order): T_s[j] = split_s(b[C - j]),
 // j in [0, L), in two copies one entry apart (copy c holds T_s[pos - c] at
-// entry pos), so every lane's 16-byte A read starts 4-byte aligned in one of
-// them.  Zero outside the taps.
+// entry pos, pos in [0, L]), so every lane's 16-byte A read starts 4-byte
+// aligned in one of them.  Zero outside the taps.  Each tap is loaded once for
+// both copies, four loads in flight per thread before any is used.
 template <int S, int TW>
 __device__ __forceinline__ void build_tap_table(unsigned char *lds, int tab_offset,
                                                 int copy_bytes, const float *__restrict__ b,
                                                 int K, int m_hi)
 {
     const int C = 16 * (m_hi + 2 * TW + 1) + 32, L = C + 32 * TW + 32;
-    for (int q = threadIdx.x; q < 2 * (L + 1); q += kThreads) {
-        const int c = q / (L + 1), pos = q - c * (L + 1), j = pos - c;
-        const int k = C - j;
-        const float v = (j >= 0 && j < L && k >= 0 && k < K) ? b[k] : 0.0f;
-        unsigned short t[3];
-        if (S == 3)
-            split3(v, t[0], t[1], t[2]);
-        else
-            t[0] = f16_bits((_Float16)v);
+    constexpr int R = 4;
+    // Entry j in [-1, L]: copy 0 at pos j (j >= 0), copy 1 at pos j + 1 (j < L).
+    for (int base = -1; base <= L; base += R * kThreads) {
+        float v[R];
 #pragma unroll
-        for (int s2 = 0; s2 < S; ++s2)
-            *reinterpret_cast<unsigned short *>(lds + tab_offset + (2 * s2 + c) * copy_bytes +
-                                                2 * pos) = t[s2];
+        for (int r = 0; r < R; ++r) {
+            const int jj = base + (int)threadIdx.x + r * kThreads, k = C - jj;
+            v[r] = (jj >= 0 && jj < L && k >= 0 && k < K) ? b[k] : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int jj = base + (int)threadIdx.x + r * kThreads;
+            if (jj > L) break;
+            unsigned short t[3];
+            if (S == 3)
+                split3(v[r], t[0], t[1], t[2]);
+            else
+                t[0] = f16_bits((_Float16)v[r]);
+#pragma unroll
+            for (int s2 = 0; s2 < S; ++s2) {
+                unsigned char *tab = lds + tab_offset + 2 * s2 * copy_bytes;
+                if (jj >= 0) *reinterpret_cast<unsigned short *>(tab + 2 * jj) = t[s2];
+                if (jj < L)
+                    *reinterpret_cast<unsigned short *>(tab + copy_bytes + 2 * (jj + 1)) = t[s2];
+            }
+        }
     }
 }
 
