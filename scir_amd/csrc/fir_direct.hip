@@ -404,6 +404,8 @@ int fir_dense_dispatch(const float *x, int64_t ldx, const float *b, int32_t K, i
 namespace {
 
 // Shared state of the three host threads of one pipelined host-tier call.
+// Reporting a failure allocates nothing (the message goes into a fixed
+// buffer), so it can run in any thread and while an exception unwinds.
 struct HostPipe {
     std::mutex m;
     std::condition_variable cv;
@@ -411,15 +413,18 @@ struct HostPipe {
     int64_t computed = 0;  // chunks whose kernel event is recorded
     std::atomic<bool> failed{false};
     int status = SCIR_HIP_OK;
-    std::string msg;
+    char msg[256] = {0};
 
-    void fail_with(int st, const std::string &what)
+    void fail_with(int st, const char *what, hipError_t e = hipSuccess) noexcept
     {
         std::lock_guard<std::mutex> g(m);
         if (!failed) {
             failed = true;
             status = st;
-            msg = what;
+            if (e != hipSuccess)
+                std::snprintf(msg, sizeof msg, "%s failed: %s", what, hipGetErrorString(e));
+            else
+                std::snprintf(msg, sizeof msg, "%s", what);
         }
         cv.notify_all();
     }
@@ -561,11 +566,6 @@ int ensure_pinned(DeviceState *st, size_t bytes)
     return SCIR_HIP_OK;
 }
 
-std::string hip_msg(const char *what, hipError_t e)
-{
-    return std::string(what) + " failed: " + hipGetErrorString(e);
-}
-
 }  // namespace
 
 // The host tier in row chunks staged through pinned memory, so that both
@@ -614,12 +614,12 @@ int host_fir_pipelined(const float *x, int64_t ldx, const float *db, int32_t K, 
     HostPipe pipe;
     const int dev = st->device;
     auto rows_of = [&](int64_t c) { return std::min(rows_per_chunk, B - c * rows_per_chunk); };
-    auto in = [&]() {
+    auto in = [&]() noexcept {
         (void)hipSetDevice(dev);  // the current device is per thread
         for (int64_t c = 0; c < nch; ++c) {
             if (pipe.failed) return;
             hipError_t e = c >= 2 ? hipEventSynchronize(ev[c - 2]) : hipSuccess;
-            if (e != hipSuccess) return pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("upload", e));
+            if (e != hipSuccess) return pipe.fail_with(SCIR_HIP_ERR_BACKEND, "upload", e);
             const int64_t r0 = c * rows_per_chunk, rows = rows_of(c);
             float *slot = pin_in[c & 1];
             const float *src = x + r0 * ldx;
@@ -628,11 +628,11 @@ int host_fir_pipelined(const float *x, int64_t ldx, const float *db, int32_t K, 
             e = hipMemcpyAsync(dx + r0 * ldd, slot, (size_t)rows * ldd * sizeof(float),
                                hipMemcpyHostToDevice, sup);
             if (e == hipSuccess) e = hipEventRecord(ev[c], sup);
-            if (e != hipSuccess) return pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("upload", e));
+            if (e != hipSuccess) return pipe.fail_with(SCIR_HIP_ERR_BACKEND, "upload", e);
             pipe.advance(pipe.uploaded, c + 1);
         }
     };
-    auto out = [&]() {
+    auto out = [&]() noexcept {
         (void)hipSetDevice(dev);
         auto enqueue = [&](int64_t c) {
             if (!pipe.wait_for(pipe.computed, c)) return false;
@@ -644,7 +644,7 @@ int host_fir_pipelined(const float *x, int64_t ldx, const float *db, int32_t K, 
                                    hipMemcpyDeviceToHost, sdown);
             if (e == hipSuccess) e = hipEventRecord(ev[2 * nch + c], sdown);
             if (e != hipSuccess) {
-                pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("download", e));
+                pipe.fail_with(SCIR_HIP_ERR_BACKEND, "download", e);
                 return false;
             }
             return true;
@@ -654,8 +654,7 @@ int host_fir_pipelined(const float *x, int64_t ldx, const float *db, int32_t K, 
             // Slot (c + 1) % 2 last held chunk c - 1, already copied out.
             if (c + 1 < nch && !enqueue(c + 1)) return;
             const hipError_t e = hipEventSynchronize(ev[2 * nch + c]);
-            if (e != hipSuccess)
-                return pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("download", e));
+            if (e != hipSuccess) return pipe.fail_with(SCIR_HIP_ERR_BACKEND, "download", e);
             if (pipe.failed) return;
             const int64_t r0 = c * rows_per_chunk, rows = rows_of(c);
             const float *slot = pin_out[c & 1];
@@ -665,21 +664,33 @@ int host_fir_pipelined(const float *x, int64_t ldx, const float *db, int32_t K, 
         }
     };
     std::thread tin, tout;
-    try {
-        tin = std::thread(in);
-        tout = std::thread(out);
-    } catch (...) {
-        pipe.fail_with(SCIR_HIP_ERR_BACKEND, "host-tier thread start failed");
-        if (tin.joinable()) tin.join();
-        throw;
-    }
+    // Whatever leaves this function (an error, or an exception from the
+    // kernel dispatch below, such as a failed host allocation), both
+    // threads are stopped and joined first, and the streams drained.
+    struct JoinOnExit {
+        HostPipe &pipe;
+        std::thread &a, &b;
+        hipStream_t s0, s1, s2;
+        ~JoinOnExit()
+        {
+            if (!a.joinable() && !b.joinable()) return;
+            pipe.fail_with(SCIR_HIP_ERR_BACKEND, "host-tier pipeline stopped");
+            if (a.joinable()) a.join();
+            if (b.joinable()) b.join();
+            (void)hipStreamSynchronize(s0);
+            (void)hipStreamSynchronize(s1);
+            (void)hipStreamSynchronize(s2);
+        }
+    } joiner{pipe, tin, tout, sup, sk, sdown};
+    tin = std::thread(in);
+    tout = std::thread(out);
     // Kernels of chunk c once its upload is enqueued (the stream waits for it).
     for (int64_t c = 0; c < nch; ++c) {
         if (!pipe.wait_for(pipe.uploaded, c)) break;
         const int64_t r0 = c * rows_per_chunk;
         hipError_t e = hipStreamWaitEvent(sk, ev[c], 0);
         if (e != hipSuccess) {
-            pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("hipStreamWaitEvent", e));
+            pipe.fail_with(SCIR_HIP_ERR_BACKEND, "hipStreamWaitEvent", e);
             break;
         }
         const int st2 = fir_dense_dispatch(dx + r0 * ldd, ldd, db, K, rows_of(c), N,
@@ -690,7 +701,7 @@ int host_fir_pipelined(const float *x, int64_t ldx, const float *db, int32_t K, 
         }
         e = hipEventRecord(ev[nch + c], sk);
         if (e != hipSuccess) {
-            pipe.fail_with(SCIR_HIP_ERR_BACKEND, hip_msg("hipEventRecord", e));
+            pipe.fail_with(SCIR_HIP_ERR_BACKEND, "hipEventRecord", e);
             break;
         }
         pipe.advance(pipe.computed, c + 1);
@@ -701,10 +712,10 @@ int host_fir_pipelined(const float *x, int64_t ldx, const float *db, int32_t K, 
     // pinned slots are reused by the next call).
     const hipError_t e1 = hipStreamSynchronize(sup), e2 = hipStreamSynchronize(sk),
                      e3 = hipStreamSynchronize(sdown);
-    if (pipe.failed) return fail(pipe.status, "%s", pipe.msg.c_str());
-    if (e1 != hipSuccess) return fail(SCIR_HIP_ERR_BACKEND, "%s", hip_msg("upload", e1).c_str());
-    if (e2 != hipSuccess) return fail(SCIR_HIP_ERR_BACKEND, "%s", hip_msg("kernels", e2).c_str());
-    if (e3 != hipSuccess) return fail(SCIR_HIP_ERR_BACKEND, "%s", hip_msg("download", e3).c_str());
+    if (pipe.failed) return fail(pipe.status, "%s", pipe.msg);
+    if (e1 != hipSuccess) return fail(SCIR_HIP_ERR_BACKEND, "upload failed: %s", hipGetErrorString(e1));
+    if (e2 != hipSuccess) return fail(SCIR_HIP_ERR_BACKEND, "kernels failed: %s", hipGetErrorString(e2));
+    if (e3 != hipSuccess) return fail(SCIR_HIP_ERR_BACKEND, "download failed: %s", hipGetErrorString(e3));
     return SCIR_HIP_OK;
 }
 
