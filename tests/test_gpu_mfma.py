@@ -123,9 +123,9 @@ def test_full_size_long_taps_rel_l2(hip, oracle, K):
     # side within (K+1) 2^-23 sum|b||x| of the exact sum; 0.1 % slack for
     # the rounding of sabs itself)
     lim = 2.0 * (K + 1) * EPS * sabs * 1.001 + 1e-30
-    excess = ((y3 - ye).abs() - lim).max().item()
-    print(f"K={K}: every-row excess over the bound {excess:.3g} (<= 0 passes)")
-    assert excess <= 0.0
+    ratio = ((y3 - ye).abs() / lim).max().item()
+    print(f"K={K}: every-row max error / bound {ratio:.3g} (<= 1 passes)")
+    assert ratio <= 1.0
     del ye, sabs
     rows = [0, 1023, 2047]
     xs = x[rows].cpu().numpy()

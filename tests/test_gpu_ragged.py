@@ -188,8 +188,11 @@ def test_graph_replay_matches_eager(hip, precision):
 
 def test_full_size_ragged_workload(hip, oracle):
     """10000 signals, lengths log-uniform in [256, 2^20], K in [15, 1023],
-    windowed-sinc low-passes: a subset of signals vs the FP64 oracle, and the
-    persistent launch vs per-signal launches over every signal."""
+    windowed-sinc low-passes: a subset of signals vs the FP64 oracle, the
+    persistent launch vs per-signal launches over every signal, and every
+    sample of every signal (FMA direct form and BF16x3) within
+    2 (K_s + 1) 2^-23 sum|b||x| of the reference loop (f32_exact), with
+    sum|b||x| computed on the GPU."""
     import torch
     from scir_amd import configs, device
     from scir_amd.ragged import RaggedPlan
@@ -201,8 +204,22 @@ def test_full_size_ragged_workload(hip, oracle):
     plan = RaggedPlan(offs, boffs, "f32")
     y = plan.execute(x, bt)
     y2 = plan.execute_per_signal(x, bt)
+    sabs = plan.execute(x.abs(), bt.abs())
     torch.cuda.synchronize()
     assert torch.equal(y, y2)
+    del y2
+    pe = RaggedPlan(offs, boffs, "f32_exact")
+    ye = pe.execute(x, bt)
+    pe.close()
+    kp1 = torch.repeat_interleave(
+        torch.tensor(np.diff(boffs) + 1, dtype=torch.float32, device="cuda"),
+        torch.tensor(np.diff(offs), device="cuda"))
+    # 0.1 % slack for the rounding of sabs itself
+    lim = 2.0 * kp1 * 2.0 ** -23 * sabs * 1.001 + 1e-30
+    del kp1, sabs
+    ratio_f32 = ((y - ye).abs() / lim).max().item()
+    print(f"ragged f32: every-sample max error / bound {ratio_f32:.3g} (<= 1 passes)")
+    assert ratio_f32 <= 1.0
     lens = np.diff(offs)
     pick = sorted(set([int(np.argmin(lens)), int(np.argmax(lens))] +
                       list(rng(9).integers(0, offs.size - 1, 10))))
@@ -220,6 +237,11 @@ def test_full_size_ragged_workload(hip, oracle):
     y2 = plan.execute_per_signal(x, bt)
     torch.cuda.synchronize()
     assert torch.equal(y, y2)
+    del y2
+    ratio_b3 = ((y - ye).abs() / lim).max().item()
+    print(f"ragged bf16x3: every-sample max error / bound {ratio_b3:.3g} (<= 1 passes)")
+    assert ratio_b3 <= 1.0
+    del ye, lim
     for s in pick:
         seg = slice(int(offs[s]), int(offs[s + 1]))
         xs = x[seg].cpu().numpy()
