@@ -398,11 +398,18 @@ def other_configs(stream):
     wl, x1, b1, b1_np = dense_setup("direct_k129")
     y1 = torch.empty_like(x1)
     ms, _ = timed(lambda: device.fir(x1, b1, "f32", out=y1), 20, 30, stream)
+    # What the drop-in's default (AUTO: BF16x3 MFMA from K = 56 up, same
+    # elementwise bound) runs on the same inputs.
+    ms_a, _ = timed(lambda: device.fir(x1, b1, "auto", out=y1), 20, 30, stream)
+    auto = dict(resolves_to="bf16x3", gsamples_s=round(wl["B"] * wl["N"] / (ms_a * 1e-3) / 1e9, 2),
+                ms_per_step=round(ms_a, 4),
+                rel_l2_vs_fp64=rel_l2_rows(x1, y1, b1_np, [0, 2048, 4095]))
+    ms, _ = timed(lambda: device.fir(x1, b1, "f32", out=y1), 20, 30, stream)
     res["c1_direct_k129"] = dict(
         rows=wl["B"], samples_per_row=wl["N"], taps=wl["K"], precision="f32",
         gsamples_s=round(wl["B"] * wl["N"] / (ms * 1e-3) / 1e9, 2), ms_per_step=round(ms, 4),
         rel_l2_vs_fp64=rel_l2_rows(x1, y1, b1_np, [0, 2048, 4095]),
-        roofline=roofline_for(wl, "direct_k129", "f32", ms * 1e-3))
+        roofline=roofline_for(wl, "direct_k129", "f32", ms * 1e-3), auto=auto)
     del x1, y1
     # c2, K = 2048: every arithmetic on the same inputs
     wl, x2, b2, b2_np = dense_setup("longtap_k2048")
