@@ -22,7 +22,7 @@ namespace scir {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kJ = 4;                     // outputs per lane
+constexpr int kJ = 8;                     // outputs per lane
 constexpr int kT = kThreads * kJ;         // outputs per workgroup
 constexpr int kMaxTaps = 8192;
 
