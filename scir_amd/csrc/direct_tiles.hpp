@@ -81,6 +81,10 @@ constexpr int kSuper = 2048;  // taps per LDS super-chunk (multiple of 32)
 constexpr int kChunk = 32;    // taps per fully unrolled step
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+// Four floats at any 4-byte aligned global address (the hardware runs in
+// unaligned access mode): rows are read and written with 16-byte accesses
+// whatever their offset or pitch.
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 template <bool EXACT>
@@ -131,16 +135,16 @@ __device__ __forceinline__ void taps_chunk(float (&acc)[R], const float *xs, int
 // Logical samples g..g+3 of a row as a float4, zero outside [0, N).  With
 // `rev` the row is read back to front (logical i == physical N-1-i), which
 // turns the causal filter into the anti-causal correlation of the adjoint.
-// `vec` promises 16-byte aligned rows (and N % 4 == 0 when rev is set).
+// One 16-byte load when the four samples are inside the row.
 __device__ __forceinline__ f32x4 load4(const float *__restrict__ xrow, int64_t g, int64_t N,
-                                       bool vec, bool rev)
+                                       bool rev)
 {
     f32x4 v;
-    if (vec && g >= 0 && g + 3 < N) {
+    if (g >= 0 && g + 3 < N) {
         if (!rev) {
-            v = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + g));
+            v = __builtin_nontemporal_load(reinterpret_cast<const f32x4u *>(xrow + g));
         } else {
-            f32x4 t = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(xrow + (N - 4 - g)));
+            f32x4 t = __builtin_nontemporal_load(reinterpret_cast<const f32x4u *>(xrow + (N - 4 - g)));
             v = f32x4{t.w, t.z, t.y, t.x};
         }
     } else {
@@ -156,11 +160,10 @@ __device__ __forceinline__ f32x4 load4(const float *__restrict__ xrow, int64_t g
 // Stages logical samples g0 .. g0 + count (count and g0 multiples of 4) into
 // xs (padded layout).
 __device__ __forceinline__ void stage_row(float *xs, const float *__restrict__ xrow, int64_t g0,
-                                          int count, int64_t N, bool vec_ok, bool rev)
+                                          int count, int64_t N, bool rev)
 {
     for (int v = threadIdx.x; v < count / 4; v += kThreads)
-        *reinterpret_cast<f32x4 *>(xs + 4 * v) =
-            load4(xrow, g0 + 4 * (int64_t)v, N, vec_ok, rev);
+        *reinterpret_cast<f32x4 *>(xs + 4 * v) = load4(xrow, g0 + 4 * (int64_t)v, N, rev);
 }
 
 // acc[r] += sum_{k' < kcount} bs[k'] * x[i_r - k'] with the window in LDS:
@@ -231,17 +234,17 @@ __device__ __forceinline__ void fix_ramp_in(float (&acc)[R], const float *__rest
 
 template <int R>
 __device__ __forceinline__ void store_outputs(const float (&acc)[R], float *__restrict__ yrow,
-                                              int64_t i0, int64_t N, bool vec_y, bool rev)
+                                              int64_t i0, int64_t N, bool rev)
 {
-    if (vec_y && i0 + R <= N) {
+    if (i0 + R <= N) {
 #pragma unroll
         for (int r = 0; r < R; r += 4) {
             if (!rev) {
                 f32x4 v = {acc[r], acc[r + 1], acc[r + 2], acc[r + 3]};
-                *reinterpret_cast<f32x4 *>(yrow + i0 + r) = v;
+                *reinterpret_cast<f32x4u *>(yrow + i0 + r) = v;
             } else {
                 f32x4 v = {acc[r + 3], acc[r + 2], acc[r + 1], acc[r]};
-                *reinterpret_cast<f32x4 *>(yrow + (N - 4 - i0 - r)) = v;
+                *reinterpret_cast<f32x4u *>(yrow + (N - 4 - i0 - r)) = v;
             }
         }
     } else {

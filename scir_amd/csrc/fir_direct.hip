@@ -53,8 +53,7 @@ using namespace direct;
 template <int R, bool EXACT>
 __global__ __launch_bounds__(kThreads) void fir_direct_kernel(
     const float *__restrict__ x, int64_t ldx, const float *__restrict__ b, int K, int64_t N,
-    float *__restrict__ y, int64_t ldy, int64_t row0, int tiles_per_row, int vec_x, int vec_y,
-    int rev)
+    float *__restrict__ y, int64_t ldy, int64_t row0, int tiles_per_row, int rev)
 {
     extern __shared__ __attribute__((aligned(16))) float xs[];
     constexpr int T = kThreads * R;
@@ -71,12 +70,12 @@ __global__ __launch_bounds__(kThreads) void fir_direct_kernel(
         const int kcount = min(kSuper, K - ks);
         const int H = (kcount + 6) & ~3;  // >= kcount + 3, multiple of 4
         if (ks > 0) __syncthreads();      // previous super-chunk done with xs
-        stage_row(xs, xrow, t0 - ks - H, T + H + 4, N, vec_x != 0, rev != 0);
+        stage_row(xs, xrow, t0 - ks - H, T + H + 4, N, rev != 0);
         __syncthreads();
         taps_from_lds<R, EXACT>(acc, xs, H, lane_base, kcount, b + ks);
     }
     fix_ramp_in<R, EXACT>(acc, xrow, N, rev != 0, b, K, t0 + lane_base);
-    store_outputs<R>(acc, y + row * ldy, t0 + lane_base, N, vec_y != 0, rev != 0);
+    store_outputs<R>(acc, y + row * ldy, t0 + lane_base, N, rev != 0);
 }
 
 // Pipelined variant for K <= kPipeMaxK: one workgroup walks `tiles_per_seg`
@@ -90,7 +89,7 @@ template <int R, bool EXACT, int NS>
 __global__ __launch_bounds__(kThreads) void fir_direct_pipe_kernel(
     const float *__restrict__ x, int64_t ldx, const float *__restrict__ b, int K, int64_t N,
     float *__restrict__ y, int64_t ldy, int64_t row0, int segs_per_row, int tiles_per_seg,
-    int64_t tiles_per_row, int buf_floats, int vec_x, int vec_y, int rev)
+    int64_t tiles_per_row, int buf_floats, int rev)
 {
     extern __shared__ __attribute__((aligned(16))) float lds[];
     constexpr int T = kThreads * R;
@@ -102,7 +101,6 @@ __global__ __launch_bounds__(kThreads) void fir_direct_pipe_kernel(
     const float *__restrict__ xrow = x + row * ldx;
     float *__restrict__ yrow = y + row * ldy;
     const int lane_base = threadIdx.x * R;
-    const bool vx = vec_x != 0;
 
     f32x4 pre[NS];
     auto fetch = [&](int64_t tile) {
@@ -110,7 +108,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_pipe_kernel(
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             const int v = threadIdx.x + s * kThreads;
-            if (v < count4) pre[s] = load4(xrow, g0 + 4 * (int64_t)v, N, vx, rev != 0);
+            if (v < count4) pre[s] = load4(xrow, g0 + 4 * (int64_t)v, N, rev != 0);
         }
     };
 
@@ -127,7 +125,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_pipe_kernel(
         float acc[R];
         taps_from_lds<R, EXACT, true>(acc, buf, H, lane_base, K, b);
         fix_ramp_in<R, EXACT>(acc, xrow, N, rev != 0, b, K, (tfirst + it) * T + lane_base);
-        store_outputs<R>(acc, yrow, (tfirst + it) * T + lane_base, N, vec_y != 0, rev != 0);
+        store_outputs<R>(acc, yrow, (tfirst + it) * T + lane_base, N, rev != 0);
     }
 }
 
@@ -263,15 +261,13 @@ int launch_direct(const float *x, int64_t ldx, const float *b, int K, int64_t B,
     const int64_t tiles = (N + T - 1) / T;
     if (tiles > 0x7fffffff)
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row too long (N=%lld)", (long long)N);
-    const int nvec = !rev || (N % 4 == 0);
-    const int vec_x = align_flags(x, ldx) && nvec, vec_y = align_flags(y, ldy) && nvec;
     // One launch covers as many rows as fit in a 2^31-1 workgroup grid.
     const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / tiles);
     for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
         const int64_t rows = std::min(rows_per_launch, B - r0);
         hipLaunchKernelGGL((fir_direct_kernel<R, EXACT>), dim3((unsigned)(rows * tiles)),
                            dim3(kThreads), lds, stream, x, ldx, b, K, N, y, ldy, r0, (int)tiles,
-                           vec_x, vec_y, (int)rev);
+                           (int)rev);
         SCIR_TRY(check_launch("fir_direct_kernel"));
     }
     return SCIR_HIP_OK;
@@ -295,14 +291,12 @@ int launch_direct_pipe(const float *x, int64_t ldx, const float *b, int K, int64
     const int64_t segs = (tiles + tps - 1) / tps;
     if (segs > 0x7fffffff)
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row too long (N=%lld)", (long long)N);
-    const int nvec = !rev || (N % 4 == 0);
-    const int vec_x = align_flags(x, ldx) && nvec, vec_y = align_flags(y, ldy) && nvec;
     const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / segs);
     for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
         const int64_t rows = std::min(rows_per_launch, B - r0);
         hipLaunchKernelGGL((fir_direct_pipe_kernel<R, EXACT, NS>), dim3((unsigned)(rows * segs)),
                            dim3(kThreads), lds, stream, x, ldx, b, K, N, y, ldy, r0, (int)segs,
-                           (int)tps, tiles, buf_floats, vec_x, vec_y, (int)rev);
+                           (int)tps, tiles, buf_floats, (int)rev);
         SCIR_TRY(check_launch("fir_direct_pipe_kernel"));
     }
     return SCIR_HIP_OK;

@@ -52,8 +52,6 @@ __global__ __launch_bounds__(kThreads) void fir_ragged_kernel(
         const RaggedItem w = items[it];
         const float *__restrict__ xrow = x + w.x0;
         float *__restrict__ yrow = y + w.x0;
-        const bool vx = (reinterpret_cast<uintptr_t>(xrow) & 15) == 0;
-        const bool vy = (reinterpret_cast<uintptr_t>(yrow) & 15) == 0;
         for (int tile = 0; tile < w.ntiles; ++tile) {
             const int64_t t0 = w.t0 + (int64_t)tile * kT;
             float acc[kR];
@@ -63,12 +61,12 @@ __global__ __launch_bounds__(kThreads) void fir_ragged_kernel(
                 const int kcount = min(kSuper, w.K - ks);
                 const int H = (kcount + 6) & ~3;
                 __syncthreads();  // previous stage finished with xs
-                stage_row(xs, xrow, t0 - ks - H, kT + H + 4, w.len, vx, false);
+                stage_row(xs, xrow, t0 - ks - H, kT + H + 4, w.len, false);
                 __syncthreads();
                 taps_from_lds<kR, EXACT>(acc, xs, H, lane_base, kcount, b + w.b0 + ks);
             }
             fix_ramp_in<kR, EXACT>(acc, xrow, w.len, false, b + w.b0, w.K, t0 + lane_base);
-            store_outputs<kR>(acc, yrow, t0 + lane_base, w.len, vy, false);
+            store_outputs<kR>(acc, yrow, t0 + lane_base, w.len, false);
         }
     }
 }
