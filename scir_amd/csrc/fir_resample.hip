@@ -22,6 +22,8 @@ namespace scir {
 namespace {
 
 constexpr int kThreads = 256;
+// Two doubles at any 8-byte aligned global address (unaligned access mode).
+typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
 constexpr int kJ = 8;                     // outputs per lane
 constexpr int kT = kThreads * kJ;         // outputs per workgroup
 constexpr int kMaxTaps = 8192;
@@ -47,25 +49,17 @@ __global__ __launch_bounds__(kThreads) void resample_poly_kernel(
     const int r0 = (int)(n0 % up);        // phase of output m0
     const double *__restrict__ xr = x + row * ldx;
     for (int k = threadIdx.x; k < K; k += kThreads) hs[k] = h[k];
-    // Stage x[s0 .. s0 + span): pairs of samples where both are inside the
-    // row and 16-byte aligned, single samples otherwise.
-    const bool pairs = ((reinterpret_cast<uintptr_t>(xr) + 8 * s0) & 15) == 0;
-    if (pairs) {
-        for (int i = 2 * threadIdx.x; i < span; i += 2 * kThreads) {
-            const int64_t s = s0 + i;
-            if (s >= 0 && s + 1 < N) {
-                const double2 v = *reinterpret_cast<const double2 *>(xr + s);
-                xs[i] = v.x;
-                if (i + 1 < span) xs[i + 1] = v.y;
-            } else {
-                xs[i] = (s >= 0 && s < N) ? xr[s] : 0.0;
-                if (i + 1 < span) xs[i + 1] = (s + 1 >= 0 && s + 1 < N) ? xr[s + 1] : 0.0;
-            }
-        }
-    } else {
-        for (int i = threadIdx.x; i < span; i += kThreads) {
-            const int64_t s = s0 + i;
+    // Stage x[s0 .. s0 + span): pairs of samples (one 16-byte load when both
+    // are inside the row), zero outside it.
+    for (int i = 2 * threadIdx.x; i < span; i += 2 * kThreads) {
+        const int64_t s = s0 + i;
+        if (s >= 0 && s + 1 < N) {
+            const f64x2u v = *reinterpret_cast<const f64x2u *>(xr + s);
+            xs[i] = v.x;
+            if (i + 1 < span) xs[i + 1] = v.y;
+        } else {
             xs[i] = (s >= 0 && s < N) ? xr[s] : 0.0;
+            if (i + 1 < span) xs[i + 1] = (s + 1 >= 0 && s + 1 < N) ? xr[s + 1] : 0.0;
         }
     }
     __syncthreads();

@@ -28,6 +28,8 @@ namespace scir {
 namespace {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+// Four floats at any 4-byte aligned global address (unaligned access mode).
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
 
 constexpr int kThreads = 256;
 constexpr int kTile = 2048;   // samples per tile
@@ -35,10 +37,9 @@ constexpr int kLagPass = 256; // lags per pass (16 lag groups x 16 lags)
 
 __device__ __forceinline__ int pad16(int s) { return s + ((s >> 4) << 2); }
 
-__device__ __forceinline__ f32x4 load4g(const float *__restrict__ row, int64_t i, int64_t N,
-                                        bool vec)
+__device__ __forceinline__ f32x4 load4g(const float *__restrict__ row, int64_t i, int64_t N)
 {
-    if (vec && i >= 0 && i + 3 < N) return *reinterpret_cast<const f32x4 *>(row + i);
+    if (i >= 0 && i + 3 < N) return *reinterpret_cast<const f32x4u *>(row + i);
     f32x4 v;
 #pragma unroll
     for (int c = 0; c < 4; ++c) v[c] = (i + c >= 0 && i + c < N) ? row[i + c] : 0.0f;
@@ -47,8 +48,7 @@ __device__ __forceinline__ f32x4 load4g(const float *__restrict__ row, int64_t i
 
 __global__ __launch_bounds__(kThreads) void fir_vjp_db_partial_kernel(
     const float *__restrict__ g, int64_t ldg, const float *__restrict__ x, int64_t ldx, int K,
-    int64_t N, int segs_per_row, int tiles_per_wg, int hist, float *__restrict__ partial,
-    int vec_g, int vec_x)
+    int64_t N, int segs_per_row, int tiles_per_wg, int hist, float *__restrict__ partial)
 {
     extern __shared__ __attribute__((aligned(16))) float sm[];
     float *gs = sm;                                 // kTile floats
@@ -74,10 +74,10 @@ __global__ __launch_bounds__(kThreads) void fir_vjp_db_partial_kernel(
             const int64_t xbase = t0 - hist;
             __syncthreads();  // previous tile done with gs/xs
             for (int v = threadIdx.x; v < kTile / 4; v += kThreads)
-                *reinterpret_cast<f32x4 *>(gs + 4 * v) = load4g(grow, t0 + 4 * v, N, vec_g);
+                *reinterpret_cast<f32x4 *>(gs + 4 * v) = load4g(grow, t0 + 4 * v, N);
             for (int v = threadIdx.x; v < (kTile + hist) / 4; v += kThreads)
                 *reinterpret_cast<f32x4 *>(xs + pad16(4 * v)) =
-                    load4g(xrow, xbase + 4 * v, N, vec_x);
+                    load4g(xrow, xbase + 4 * v, N);
             __syncthreads();
             float acc[16];
 #pragma unroll
@@ -144,8 +144,7 @@ constexpr int kMfmaLook = 256;  // g samples staged past the tile (16j + l < 256
 
 __global__ __launch_bounds__(kThreads) void fir_vjp_db_mfma_kernel(
     const float *__restrict__ g, int64_t ldg, const float *__restrict__ x, int64_t ldx, int K,
-    int64_t N, int segs_per_row, int tiles_per_wg, float *__restrict__ partial, int vec_g,
-    int vec_x)
+    int64_t N, int segs_per_row, int tiles_per_wg, float *__restrict__ partial)
 {
     constexpr int GS = kTile + kMfmaLook;  // staged g samples
     constexpr int XS = kTile + 32;         // staged x samples
@@ -170,12 +169,12 @@ __global__ __launch_bounds__(kThreads) void fir_vjp_db_mfma_kernel(
 #pragma unroll
         for (int u = 0; u < NG; ++u) {
             const int v = threadIdx.x + u * kThreads;
-            if (v < GS / 4) pg[u] = load4g(grow, t0 + 4 * v, N, vec_g);
+            if (v < GS / 4) pg[u] = load4g(grow, t0 + 4 * v, N);
         }
 #pragma unroll
         for (int u = 0; u < NX; ++u) {
             const int v = threadIdx.x + u * kThreads;
-            if (v < XS / 4) px[u] = load4g(xrow, t0 - kp - 16 + 4 * v, N, vec_x);
+            if (v < XS / 4) px[u] = load4g(xrow, t0 - kp - 16 + 4 * v, N);
         }
     };
     for (int kp = 0; kp < K; kp += kLagPass) {
@@ -270,11 +269,6 @@ __global__ __launch_bounds__(kThreads) void fir_vjp_db_final_kernel(
     }
 }
 
-int aligned16(const float *p, int64_t ld)
-{
-    return ((reinterpret_cast<uintptr_t>(p) & 15) == 0) && (ld % 4 == 0);
-}
-
 // Partial sums of the allocation-free dL/db entry point: a file-scope device
 // array, one copy per GPU, held until the process ends.  Host code never
 // passes `scratch` itself to a kernel (its name is a host-side stand-in);
@@ -353,8 +347,7 @@ int vjp_db_run(const float *g, int64_t ldg, const float *x, int64_t ldx, int32_t
             partial = static_cast<float *>(sc.partials.ptr);
         }
         hipLaunchKernelGGL(fir_vjp_db_mfma_kernel, dim3((unsigned)nwg), dim3(kThreads), 0, s, g,
-                           ldg, x, ldx, (int)K, N, (int)segs, (int)tiles_per_wg, partial,
-                           aligned16(g, ldg), aligned16(x, ldx));
+                           ldg, x, ldx, (int)K, N, (int)segs, (int)tiles_per_wg, partial);
         SCIR_TRY(check_launch("fir_vjp_db_mfma_kernel"));
         hipLaunchKernelGGL(fir_vjp_db_final_kernel, dim3((K + kFinalLags - 1) / kFinalLags),
                            dim3(kThreads), 0, s, partial, nwg, (int)K, db);
@@ -373,8 +366,7 @@ int vjp_db_run(const float *g, int64_t ldg, const float *x, int64_t ldx, int32_t
     SCIR_TRY(ensure_scratch(st, sc.partials, (size_t)nwg * K * sizeof(float), s));
     float *partial = static_cast<float *>(sc.partials.ptr);
     hipLaunchKernelGGL(fir_vjp_db_partial_kernel, dim3((unsigned)nwg), dim3(kThreads), lds, s, g,
-                       ldg, x, ldx, (int)K, N, (int)segs, (int)tiles_per_wg, hist, partial,
-                       aligned16(g, ldg), aligned16(x, ldx));
+                       ldg, x, ldx, (int)K, N, (int)segs, (int)tiles_per_wg, hist, partial);
     SCIR_TRY(check_launch("fir_vjp_db_partial_kernel"));
     hipLaunchKernelGGL(fir_vjp_db_final_kernel, dim3((K + kFinalLags - 1) / kFinalLags),
                        dim3(kThreads), 0, s, partial, nwg, (int)K, db);
