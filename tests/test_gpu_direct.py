@@ -200,3 +200,29 @@ def test_auto_dispatch_hip_device(hip, oracle):
     y = scir_amd.fir1d_batched_f32_auto(x, taps, scir_amd.Device.Hip)
     y64 = oracle.fir_ref_f64(x.astype(np.float64), taps.astype(np.float64))
     assert np.all(np.abs(y - y64) <= oracle.fir_error_bound(x, taps[::-1]))
+
+
+@pytest.mark.parametrize("precision", ["f32", "f32_exact", "bf16x3", "fp16", "fft32"])
+@pytest.mark.parametrize("ox,oy", [(1, 2), (3, 1), (2, 3), (1, 0)])
+def test_unaligned_rows_equal_aligned(hip, precision, ox, oy):
+    """Rows at any 4-byte offset and odd pitch give the same bits as the same
+    rows stored aligned and packed (for the FP32 forms: the register-prefetch
+    kernel, which takes unaligned rows, against the LDS-DMA kernel, which
+    takes aligned ones), and nothing outside y is written."""
+    import torch
+    from scir_amd import device
+    B, N, K = 6, 10000, 129
+    print(f"unaligned rows {precision} offsets x={ox} y={oy}", flush=True)
+    r = rng(ox * 10 + oy)
+    xa = torch.tensor(r.standard_normal((B, N)).astype(np.float32), device="cuda")
+    b = torch.tensor((r.standard_normal(K) / np.sqrt(K)).astype(np.float32), device="cuda")
+    ya = device.fir(xa, b, precision)
+    xbase = torch.full((B, N + 5), float("nan"), device="cuda")
+    xbase[:, ox:ox + N] = xa
+    ybase = torch.full((B, N + 7), float("nan"), device="cuda")
+    yu = ybase[:, oy:oy + N]
+    device.fir(xbase[:, ox:ox + N], b, precision, out=yu)
+    torch.cuda.synchronize()
+    assert torch.equal(ya, yu), precision
+    yb = ybase.cpu().numpy()
+    assert np.isnan(yb[:, :oy]).all() and np.isnan(yb[:, oy + N:]).all()

@@ -70,3 +70,35 @@ def test_full_size_rows(hip, oracle):
     for i, r in enumerate(rows):
         ref = oracle.resample_poly_f64(xs[i], 2, 3, h)
         assert np.all(np.abs(ys[i] - ref) <= bound(oracle, xs[i], 2, 3, h)[0])
+
+
+@pytest.mark.parametrize("ox,oy", [(1, 0), (1, 1), (0, 1)])
+def test_unaligned_rows_equal_aligned(hip, ox, oy):
+    """Rows of x and y at odd 8-byte offsets with odd pitches give the same
+    bits as aligned packed rows, and nothing outside y is written."""
+    import torch
+    import ctypes
+    from scir_amd import _lib
+    B, N, up, down = 7, 6001, 2, 3
+    h = signal.RESAMPLE_H_2_3
+    M = signal.resample_len(N, up, down, h.size)
+    print(f"resample unaligned rows offsets x={ox} y={oy}", flush=True)
+    xa = torch.tensor(rng(ox * 2 + oy).standard_normal((B, N)), device="cuda")
+    hd = torch.tensor(h, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+
+    def call(x, ldx, y, ldy):
+        _lib.check(_lib.lib().scir_hip_resample_poly_f64_dev(
+            ctypes.c_void_p(x.data_ptr()), ldx, B, N, up, down, ctypes.c_void_p(hd.data_ptr()),
+            h.size, ctypes.c_void_p(y.data_ptr()), ldy, ctypes.c_void_p(st)))
+
+    ya = torch.empty((B, M), dtype=torch.float64, device="cuda")
+    call(xa, N, ya, M)
+    xbase = torch.full((B, N + 3), float("nan"), dtype=torch.float64, device="cuda")
+    xbase[:, ox:ox + N] = xa
+    ybase = torch.full((B, M + 5), float("nan"), dtype=torch.float64, device="cuda")
+    call(xbase[:, ox:], N + 3, ybase[:, oy:], M + 5)
+    torch.cuda.synchronize()
+    assert torch.equal(ya, ybase[:, oy:oy + M])
+    yb = ybase.cpu().numpy()
+    assert np.isnan(yb[:, :oy]).all() and np.isnan(yb[:, oy + M:]).all()

@@ -211,3 +211,30 @@ def test_host_tier_vjp(hip, oracle, precision):
     dx0, dt0 = fir1d_batched_f32_vjp_hip(np.zeros((2, 0), np.float32), taps,
                                          np.zeros((2, 0), np.float32))
     assert dx0.shape == (2, 0) and not np.any(dt0)
+
+
+@pytest.mark.parametrize("og,ox", [(1, 2), (3, 0), (2, 3)])
+def test_unaligned_rows_equal_aligned(hip, og, ox):
+    """g and x rows at 4-byte offsets with odd pitches give the same bits as
+    aligned packed copies, for dL/dx (FP32, BF16x3) and dL/db (FP32 MFMA form)."""
+    import torch
+    from scir_amd import device
+    B, N, K = 5, 7001, 65
+    print(f"vjp unaligned rows offsets g={og} x={ox}", flush=True)
+    r = rng(og * 10 + ox)
+    ga = torch.tensor(r.standard_normal((B, N)).astype(np.float32), device="cuda")
+    xa = torch.tensor(r.standard_normal((B, N)).astype(np.float32), device="cuda")
+    b = torch.tensor((r.standard_normal(K) / np.sqrt(K)).astype(np.float32), device="cuda")
+    gbase = torch.full((B, N + 5), float("nan"), device="cuda")
+    gbase[:, og:og + N] = ga
+    xbase = torch.full((B, N + 3), float("nan"), device="cuda")
+    xbase[:, ox:ox + N] = xa
+    gu, xu = gbase[:, og:og + N], xbase[:, ox:ox + N]
+    for precision in ("f32", "bf16x3"):
+        ybase = torch.full((B, N + 7), float("nan"), device="cuda")
+        device.vjp_dx(gu, b, precision, out=ybase[:, ox:ox + N])
+        torch.cuda.synchronize()
+        assert torch.equal(device.vjp_dx(ga, b, precision), ybase[:, ox:ox + N]), precision
+        yb = ybase.cpu().numpy()
+        assert np.isnan(yb[:, :ox]).all() and np.isnan(yb[:, ox + N:]).all()
+    assert torch.equal(device.vjp_db(ga, xa, K), device.vjp_db(gu, xu, K))
