@@ -52,6 +52,7 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0   # dense bf16/fp16 MFMA, MI355X
 FP32_PEAK_TFLOPS = 157.3
+F32_MFMA_MAX_K = 257  # the F32 mode's FP32-MFMA kernel takes K up to here
 
 METRIC = "FIR Gsamples/s & %roofline: bf16x3 MFMA vs FP32 direct at rel L2 err<=1e-6"
 
@@ -301,6 +302,19 @@ def roofline_for(wl, name, prec, avg_launch_s):
                     ceiling_note=f"{prods} MFMA products x {mfma_steps(wl['K'])} half-steps of "
                                  "32x32x16 per 32x32 output tile at 2.5 PFLOP/s")
     flops = 2.0 * wl["K"] * samples
+    # F32 up to K = 257 runs on the FP32 matrix cores (a 16 x 16(NG) Toeplitz
+    # block per 256 outputs, NG = ceil((K-1)/16) + 1); FP32 MFMA and FP32
+    # vector share the 157.3 TFLOP/s peak on MI355X.
+    f32_mfma = prec == "f32" and wl["K"] <= F32_MFMA_MAX_K
+    extra = {}
+    if f32_mfma:
+        ng = (wl["K"] + 14) // 16 + 1
+        issued = 32.0 * ng * samples / avg_launch_s / 1e12
+        extra = dict(kernel="fir_direct_mfma_kernel (FP32 MFMA, v_mfma_f32_16x16x4_f32)",
+                     mfma_issued_tflops=round(issued, 1),
+                     mfma_issued_frac=round(issued / FP32_PEAK_TFLOPS, 4),
+                     mfma_busy_frac=load_profile(f"fir_direct_{prec}_{name}_util",
+                                                 "mfma_busy_frac"))
     if prec == "fft32" or flops / (8.0 * samples) < 78.6 / 8.0:
         # HBM-bound (the FFT path's work per sample is O(log K))
         alg = 8.0 * samples
@@ -308,11 +322,11 @@ def roofline_for(wl, name, prec, avg_launch_s):
         fam = "fft" if prec == "fft32" else "direct"
         return dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4),
-                    traffic=load_traffic(f"fir_{fam}_{prec}_{name}_pmc"))
+                    traffic=load_traffic(f"fir_{fam}_{prec}_{name}_pmc"), **extra)
     ach = flops / avg_launch_s / 1e12
     return dict(bound="fp32", achieved=round(ach, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                 frac=round(ach / FP32_PEAK_TFLOPS, 4),
-                traffic=load_traffic(f"fir_direct_{prec}_{name}_pmc"))
+                traffic=load_traffic(f"fir_direct_{prec}_{name}_pmc"), **extra)
 
 
 def run_dense(args):
@@ -398,10 +412,19 @@ def other_configs(stream):
     wl, x1, b1, b1_np = dense_setup("direct_k129")
     y1 = torch.empty_like(x1)
     ms, _ = timed(lambda: device.fir(x1, b1, "f32", out=y1), 20, 30, stream)
-    # What the drop-in's default (AUTO: BF16x3 MFMA from K = 56 up, same
-    # elementwise bound) runs on the same inputs.
+    # What the drop-in's default (AUTO: the faster of F32 and BF16x3 MFMA by
+    # K, same elementwise bound) runs on the same inputs; which one it took
+    # is read off its output, bit for bit.
     ms_a, _ = timed(lambda: device.fir(x1, b1, "auto", out=y1), 20, 30, stream)
-    auto = dict(resolves_to="bf16x3", gsamples_s=round(wl["B"] * wl["N"] / (ms_a * 1e-3) / 1e9, 2),
+    ya = y1.clone()
+    resolved = "other"
+    for cand in ("f32", "bf16x3"):
+        if torch.equal(ya, device.fir(x1, b1, cand)):
+            resolved = cand
+            break
+    del ya
+    auto = dict(resolves_to=resolved,
+                gsamples_s=round(wl["B"] * wl["N"] / (ms_a * 1e-3) / 1e9, 2),
                 ms_per_step=round(ms_a, 4),
                 rel_l2_vs_fp64=rel_l2_rows(x1, y1, b1_np, [0, 2048, 4095]))
     ms, _ = timed(lambda: device.fir(x1, b1, "f32", out=y1), 20, 30, stream)

@@ -71,8 +71,13 @@ typedef enum scir_hip_status {
 
 /* ---- arithmetic selection --------------------------------------------- */
 typedef enum scir_hip_precision {
-    /* FP32 direct form, fused multiply-add, any summation order; error within
-     * (K+1) * 2^-23 * sum_k |b_k||x_{n-k}| of the exact sum. */
+    /* FP32 arithmetic, any summation order; error within
+     * (K+1) * 2^-23 * sum_k |b_k||x_{n-k}| of the exact sum.  K <= 257 runs
+     * as an FP32 Toeplitz contraction on the matrix cores
+     * (v_mfma_f32_16x16x4_f32, twice the non-packed VALU FMA rate), longer
+     * filters as the direct form with fused multiply-adds on the VALU.  Both
+     * recompute any non-finite output from the reference loop's terms, so
+     * outputs are non-finite exactly where the reference loop's are. */
     SCIR_HIP_PREC_F32 = 0,
     /* FP32 direct form with a separately rounded multiply and add, taps
      * accumulated in the reference's order (k = 0, 1, ..., K-1 starting from
@@ -352,14 +357,19 @@ SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
  *   "direct.loader"   0 = one tile per workgroup, 1 = pipelined row segments
  *                     (register prefetch), 2 = pipelined row segments copied
  *                     into LDS by DMA two tiles ahead (forward, 16-byte aligned
- *                     rows with N % 4 == 0; other calls use 1)
+ *                     rows with N % 4 == 0; other calls use 1), 3 (default) =
+ *                     F32 on FP32 MFMA for K <= 257 (F32_EXACT and longer
+ *                     filters as 2)
+ *   "direct.mfma_strip" 256-output tiles per wave of loader 3 (1 .. 4096;
+ *                     default 8 for K <= 64, else 32)
  *   "direct.seg_tiles" tiles per workgroup of the pipelined direct kernel
  *   "direct.dma_bufs" LDS tile buffers of loader 2 (2, 3 or 4: 1 to 3 tiles in flight)
  *   "mfma.tiles"      32x32 output tiles per wave of the MFMA kernel (1, 2, 4)
  *   "mfma.segs"       row segments per workgroup of the MFMA kernel (>= 1); the
  *                     next segment's samples load while the current one computes
  *   "vjp.db_mfma"     1 (default): dL/db on FP32 MFMA; 0: the FP32 VALU kernel
- *   "auto.mfma_min_k" smallest K for which AUTO picks BF16x3 MFMA
+ *   "auto.mfma_min_k" smallest K for which AUTO picks BF16x3 MFMA (default
+ *                     130: F32 on FP32 MFMA is faster below)
  *   "auto.fft"        1: AUTO may pick the FFT32 path (normwise error) from
  *                     auto.fft_min_k up (x != y); 0 (default): never
  *   "auto.fft_min_k"  smallest K for which AUTO (with auto.fft) picks FFT32;

@@ -50,6 +50,7 @@ int g_over_rg_direct_wg = -1, g_over_rg_direct_kmacs = -1, g_over_rg_mfma_segs =
     g_over_rg_mfma_wg = -1;
 int g_over_host_chunk_kib = -1;
 int g_over_host_copy_threads = -1;
+int g_over_direct_mfma_strip = -1;
 
 // Buckets searched by the autotuner.
 const int kKBuckets[][2] = {{1, 16}, {17, 64}, {65, 256}, {257, 1017}, {1018, 4096}};
@@ -78,6 +79,7 @@ TuneChoice tune_lookup(int K, int64_t N)
     if (g_over_direct_loader >= 0) c.direct_loader = g_over_direct_loader;
     if (g_over_direct_seg >= 0) c.direct_seg_tiles = g_over_direct_seg;
     if (g_over_dma_bufs >= 0) c.direct_dma_bufs = g_over_dma_bufs;
+    if (g_over_direct_mfma_strip >= 0) c.direct_mfma_strip = g_over_direct_mfma_strip;
     if (g_over_mfma_tiles >= 0) c.mfma_tiles = c.mfma_tiles_fp16 = g_over_mfma_tiles;
     if (g_over_auto_min_k >= 0) c.auto_mfma_min_k = g_over_auto_min_k;
     if (g_over_mfma_segs >= 0) {
@@ -156,12 +158,16 @@ extern "C" int scir_hip_tune_set(const char *key, int32_t value)
                 return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "direct.r must be 8 or 16");
             g_over_direct_r = v;
         } else if (!std::strcmp(key, "direct.loader")) {
-            if (v > 2) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "direct.loader must be 0, 1 or 2");
+            if (v > 3) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "direct.loader must be 0 .. 3");
             g_over_direct_loader = v;
         } else if (!std::strcmp(key, "direct.dma_bufs")) {
             if (v >= 0 && (v < 2 || v > 4))
                 return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "direct.dma_bufs must be 2, 3 or 4");
             g_over_dma_bufs = v;
+        } else if (!std::strcmp(key, "direct.mfma_strip")) {
+            if (v == 0 || v > 4096)
+                return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "direct.mfma_strip: 1 .. 4096");
+            g_over_direct_mfma_strip = v;
         } else if (!std::strcmp(key, "direct.seg_tiles")) {
             if (v == 0) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "direct.seg_tiles must be >= 1");
             g_over_direct_seg = v;
@@ -289,9 +295,10 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
                 status = fail(SCIR_HIP_ERR_BACKEND, "autotune tap upload failed");
         }
         // Search with the table cleared; overrides drive each candidate.
-        int saved[6];
+        int saved[7];
         {
             std::lock_guard<std::mutex> g(g_tune_mu);
+            saved[6] = g_over_direct_mfma_strip;
             saved[0] = g_over_direct_r;
             saved[1] = g_over_direct_loader;
             saved[2] = g_over_direct_seg;
@@ -301,8 +308,9 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
             g_table.clear();
             g_auto_min_k = -1;
         }
-        auto set = [&](int r, int loader, int seg, int tiles, int msegs) {
+        auto set = [&](int r, int loader, int seg, int tiles, int msegs, int strip = -1) {
             std::lock_guard<std::mutex> g(g_tune_mu);
+            g_over_direct_mfma_strip = strip;
             g_over_direct_r = r;
             g_over_direct_loader = loader;
             g_over_direct_seg = seg;
@@ -323,12 +331,19 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
                 e.n_lo = nb[0];
                 e.n_hi = nb[1];
                 float best = 1e30f;
-                const int rs[2] = {8, 16}, loaders[3] = {0, 1, 2}, segs[4] = {1, 2, 4, 8};
+                // Loader 3 (FP32 MFMA) has no outputs-per-lane choice; its
+                // four candidates are strips of 8, 16, 32 and 64 tiles.
+                const int rs[2] = {8, 16}, loaders[4] = {0, 1, 2, 3}, segs[4] = {1, 2, 4, 8};
                 for (int r : rs)
                     for (int ld : loaders)
                         for (int sg : segs) {
-                            if ((ld == 0 && sg != 1) || status != SCIR_HIP_OK) continue;
-                            set(r, ld, sg, -1, -1);
+                            if ((ld == 0 && sg != 1) || (ld == 3 && r != 8) ||
+                                status != SCIR_HIP_OK)
+                                continue;
+                            if (ld == 3)
+                                set(r, ld, -1, -1, -1, 8 * sg);
+                            else
+                                set(r, ld, sg, -1, -1);
                             float ms = 0;
                             status = time_ms([&]() {
                                 return scir_hip_fir_f32_dev(x, N, b, K, B, N, y, N,
@@ -338,7 +353,10 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
                                 best = ms;
                                 e.c.direct_r = r;
                                 e.c.direct_loader = ld;
-                                e.c.direct_seg_tiles = sg;
+                                if (ld == 3)
+                                    e.c.direct_mfma_strip = 8 * sg;
+                                else
+                                    e.c.direct_seg_tiles = sg;
                             }
                         }
                 const int tiles[3] = {1, 2, 4};
@@ -511,6 +529,7 @@ extern "C" int scir_hip_autotune(const char *cache_path, int32_t budget_ms)
             g_over_mfma_tiles = saved[3];
             g_over_auto_min_k = saved[4];
             g_over_mfma_segs = saved[5];
+            g_over_direct_mfma_strip = saved[6];
             if (status == SCIR_HIP_OK) {
                 g_table = found;
                 g_auto_min_k = cut;

@@ -42,6 +42,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 namespace scir {
@@ -243,6 +244,232 @@ __global__ __launch_bounds__(kThreads) void fir_direct_dma_kernel(
     }
 }
 
+// ---- F32 mode on the matrix cores (direct.loader = 3, K <= kMfmaF32MaxK) ----
+//
+// v_mfma_f32_16x16x4_f32 issues 1024 FP32 multiply-adds per 32 cycles,
+// twice the rate of non-packed FP32 FMAs on the VALU.  A tile of 256
+// consecutive outputs y[s + 16j + i] (i, j < 16) is the Toeplitz product
+//   D[i][j] = sum_c b[i - c] x[s + 16j + c],  c in [-16(NG-1), 16),
+// NG = ceil((K-1)/16) + 1, b[k] = 0 outside [0, K).  Step (J, v), J < NG,
+// v < 4, takes c = -16J + 4g + v for the MFMA reduction index g < 4:
+//   A[i][g] = b[i + 16J - 4g - v]          taps, in registers for the wave
+//   B[g][j] = x[s + 16(j - J) + 4g + v]    samples
+// Lane (g, j) (g = lane/16, j = lane%16) loads x[s + 16j + 4g .. +3] once
+// per tile (the wave's 64 16-byte loads are the tile), which is B of step
+// (0, v) in component v.  Step (J, v) takes component v of lane (g, j - J):
+// a DPP row shift by J within the lane's row of 16, lanes j < J reading the
+// previous tile's registers (DPP row rotate); J = 16 is the previous tile
+// itself.  Every (tap, output) pair of the filter occurs once; the rest of
+// the 16 x 16NG block multiplies zero taps.  D comes back in lane (g, j) as
+// y[s + 16j + 4g .. +3]: one 16-byte store.
+//
+// Each wave streams a strip of consecutive tiles of one row, loading three
+// tiles ahead, and writes with non-temporal stores (measured faster than
+// cached stores, and than non-temporal loads); no LDS, no barriers.  Per output: 16NG multiply-adds on the
+// matrix cores (48 at K = 33, 144 at K = 129) against K FMAs on the VALU,
+// which is left the DPP moves.  FP32 sums of the exact terms (and zeros)
+// with FP32 rounding, within the F32 mode's (K+1)*2^-23*sum|b||x|.  A
+// non-finite output (an Inf or NaN times a zero tap of the block, a
+// non-finite tap times a zero before the row start) is recomputed from the
+// reference loop's terms, so non-finite outputs sit where the reference
+// loop's do.
+constexpr int kMfmaF32MaxK = 257;  // NG <= 17: history from the previous tile only
+constexpr int kMfmaTile = 256;
+
+// Sample x[s + 16(j - J) + 4g + v] for lane (g, j), from component v of the
+// current tile (cur) and the previous one (prev).
+template <int J>
+__device__ __forceinline__ float history_sample(float cur, float prev)
+{
+    if constexpr (J == 0) {
+        return cur;
+    } else if constexpr (J == 16) {
+        return prev;
+    } else {
+        // row_ror:J -- lane j reads lane (j - J) mod 16 of prev; then
+        // row_shr:J -- lanes j >= J read lane j - J of cur, the others keep
+        // the rotated prev (no source lane: the write is skipped).
+        const int p = __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, prev), 0x120 + J, 0xF,
+                                               0xF, false);
+        return __builtin_bit_cast(
+            float, __builtin_amdgcn_update_dpp(p, __builtin_bit_cast(int, cur), 0x110 + J, 0xF,
+                                               0xF, false));
+    }
+}
+
+template <int J>
+__device__ __forceinline__ void toeplitz_step(f32x4 &acc, const float (&a)[4], const f32x4 &cur,
+                                              const f32x4 &prev)
+{
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], history_sample<J>(cur.x, prev.x), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], history_sample<J>(cur.y, prev.y), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], history_sample<J>(cur.z, prev.z), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], history_sample<J>(cur.w, prev.w), acc, 0, 0, 0);
+}
+
+template <int NG, int... Js>
+__device__ __forceinline__ void toeplitz_tile(f32x4 &acc, const float (&a)[NG][4], const f32x4 &cur,
+                                              const f32x4 &prev, std::integer_sequence<int, Js...>)
+{
+    (toeplitz_step<Js>(acc, a[Js], cur, prev), ...);
+}
+
+// Every output of lane chunk `off` in tiles t0 .. t0+nt-1 of a row, from the
+// reference loop's terms (fused multiply-add), overwriting what the tile
+// loop stored: the slow path of a lane that met a non-finite output, taken
+// after the strip so it adds nothing to the tile loop's registers.
+__device__ __forceinline__ void recompute_strip(const float *__restrict__ xrow,
+                                                float *__restrict__ yrow, int64_t N, bool rev,
+                                                const float *__restrict__ b, int K, int64_t t0,
+                                                int nt, int off)
+{
+    for (int m = 0; m < nt; ++m)
+        for (int r = 0; r < 4; ++r) {
+            const int64_t i = (t0 + m) * kMfmaTile + off + r;
+            if (i >= N) break;
+            float acc = 0.0f;
+            const int64_t kend = i < (int64_t)K - 1 ? i : (int64_t)K - 1;
+            for (int64_t k = 0; k <= kend; ++k)
+                acc = __builtin_fmaf(b[k], xrow[rev ? N - 1 - (i - k) : i - k], acc);
+            yrow[rev ? N - 1 - i : i] = acc;
+        }
+}
+
+template <int NG, bool REV>
+__global__ __launch_bounds__(kThreads) void fir_direct_mfma_kernel(
+    const float *__restrict__ x, int64_t ldx, const float *__restrict__ b, int K, int64_t N,
+    float *__restrict__ y, int64_t ldy, int64_t row0, int64_t strips, int64_t strips_per_row,
+    int strip_tiles)
+{
+    // The wave index is uniform: keep it (and the strip bounds) scalar.
+    const int64_t strip = (int64_t)blockIdx.x * (kThreads / 64) +
+                          __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (strip >= strips) return;  // whole waves: nothing below synchronises
+    const int64_t row = row0 + strip / strips_per_row;
+    const int64_t t0 = (strip % strips_per_row) * (int64_t)strip_tiles;
+    const int64_t tiles_in_row = (N + kMfmaTile - 1) / kMfmaTile;
+    const int nt = (int)min<int64_t>((int64_t)strip_tiles, tiles_in_row - t0);
+    const int lane = threadIdx.x & 63, li = lane & 15, lq = lane >> 4;
+    const float *__restrict__ xrow = x + row * ldx;
+    float *__restrict__ yrow = y + row * ldy;
+    constexpr bool rv = REV;  // a compile-time direction keeps the loads' results unwaited
+
+    // A of step (J, v) in lane (g = lq, i = li): b[i + 16J - 4g - v].
+    float a[NG][4];
+#pragma unroll
+    for (int J = 0; J < NG; ++J)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int k = li + 16 * J - 4 * lq - v;
+            a[J][v] = (k >= 0 && k < K) ? b[k] : 0.0f;
+        }
+
+    const int off = 16 * li + 4 * lq;  // the lane's chunk of a tile
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    // Ring of D + 2 tiles: tile t0 + m sits in q[(m + 1) % RING].  Step m
+    // reads tiles m - 1 and m and loads tile m + D into the slot of tile
+    // m - 2; the loop is unrolled by RING so no register is copied (a copy
+    // would wait for the load in flight).
+    constexpr int D = 3;
+    constexpr int RING = D + 2;
+    f32x4 q[RING];
+    q[0] = load4(xrow, (t0 - 1) * kMfmaTile + off, N, rv);  // zeros before the row
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        q[d + 1] = d < nt ? load4(xrow, (t0 + d) * kMfmaTile + off, N, rv) : zero;
+    q[RING - 1] = zero;
+    // Strips wholly inside the row load and store with plain 16-byte
+    // accesses: a bounds-checked load is a branch, and at its join the
+    // compiler waits for every load in flight.
+    const bool inside = (t0 + nt) * kMfmaTile <= N;
+    bool nonfinite = false;  // a non-finite output in this lane's chunks
+    auto step = [&](auto whole, int m, f32x4 &prev, f32x4 &cur, f32x4 &slot) {
+        constexpr bool W = decltype(whole)::value;
+        if (W) {
+            // Unconditional (a branch around a load is a join that waits):
+            // past the strip's end the last tile is loaded again, unused.
+            const int64_t g = (t0 + min(m + D, nt - 1)) * kMfmaTile + off;
+            const float *p = rv ? xrow + (N - 4 - g) : xrow + g;
+            const f32x4 t = *reinterpret_cast<const f32x4u *>(p);
+            slot = rv ? f32x4{t.w, t.z, t.y, t.x} : t;
+        } else if (m + D < nt) {
+            slot = load4(xrow, (t0 + m + D) * kMfmaTile + off, N, rv);
+        }
+        f32x4 acc = zero;
+        toeplitz_tile<NG>(acc, a, cur, prev, std::make_integer_sequence<int, NG>{});
+        float o[4] = {acc.x, acc.y, acc.z, acc.w};
+        const int64_t i0 = (t0 + m) * kMfmaTile + off;
+        // Past the row end o is 0 times zeros or a non-finite tap: ignored.
+        const bool fin = __builtin_isfinite(o[0]) & __builtin_isfinite(o[1]) &
+                         __builtin_isfinite(o[2]) & __builtin_isfinite(o[3]);
+        nonfinite |= !fin & (i0 < N);
+        if (W) {
+            f32x4u *p = reinterpret_cast<f32x4u *>(rv ? yrow + (N - 4 - i0) : yrow + i0);
+            const f32x4 v = rv ? f32x4{o[3], o[2], o[1], o[0]} : f32x4{o[0], o[1], o[2], o[3]};
+            __builtin_nontemporal_store(v, p);
+        } else {
+            store_outputs<4>(o, yrow, i0, N, rv);
+        }
+    };
+    auto run = [&](auto whole) {
+        for (int m = 0; m < nt; m += RING) {
+#pragma unroll
+            for (int u = 0; u < RING; ++u) {
+                if (m + u >= nt) break;
+                step(whole, m + u, q[u], q[(u + 1) % RING], q[(u + D + 1) % RING]);
+            }
+        }
+    };
+    if (inside)
+        run(std::true_type{});
+    else
+        run(std::false_type{});
+    if (nonfinite) recompute_strip(xrow, yrow, N, rv, b, K, t0, nt, off);
+}
+
+template <int NG, bool REV>
+int launch_direct_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
+                       float *y, int64_t ldy, int strip_tiles, hipStream_t stream)
+{
+    constexpr int64_t kWavesPerWg = kThreads / 64;
+    const int64_t tiles = (N + kMfmaTile - 1) / kMfmaTile;
+    const int64_t st = std::max<int64_t>(1, std::min<int64_t>(strip_tiles, tiles));
+    const int64_t spr = (tiles + st - 1) / st;
+    // One launch covers as many rows as fit in a 2^31-1 workgroup grid.
+    const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffffLL * kWavesPerWg / spr);
+    if (spr > 0x7fffffffLL * kWavesPerWg)
+        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "row too long (N=%lld)", (long long)N);
+    for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
+        const int64_t rows = std::min(rows_per_launch, B - r0);
+        const int64_t strips = rows * spr;
+        hipLaunchKernelGGL((fir_direct_mfma_kernel<NG, REV>),
+                           dim3((unsigned)((strips + kWavesPerWg - 1) / kWavesPerWg)),
+                           dim3(kThreads), 0, stream, x, ldx, b, K, N, y, ldy, r0, strips, spr,
+                           (int)st);
+        SCIR_TRY(check_launch("fir_direct_mfma_kernel"));
+    }
+    return SCIR_HIP_OK;
+}
+
+int launch_direct_mfma_k(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
+                         float *y, int64_t ldy, int strip_tiles, bool rev, hipStream_t stream)
+{
+    const int ng = (K + 14) / 16 + 1;  // ceil((K-1)/16) + 1
+    switch (ng) {
+#define SCIR_NG(G) \
+    case G:                                                                               \
+        return rev ? launch_direct_mfma<G, true>(x, ldx, b, K, B, N, y, ldy, strip_tiles, stream) \
+                   : launch_direct_mfma<G, false>(x, ldx, b, K, B, N, y, ldy, strip_tiles, stream);
+        SCIR_NG(1) SCIR_NG(2) SCIR_NG(3) SCIR_NG(4) SCIR_NG(5) SCIR_NG(6) SCIR_NG(7) SCIR_NG(8)
+        SCIR_NG(9) SCIR_NG(10) SCIR_NG(11) SCIR_NG(12) SCIR_NG(13) SCIR_NG(14) SCIR_NG(15)
+        SCIR_NG(16) SCIR_NG(17)
+#undef SCIR_NG
+    default:
+        return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "FP32 MFMA direct kernel needs K <= %d",
+                    kMfmaF32MaxK);
+    }
+}
+
 constexpr int kPipeMaxK = 1017;  // (K + 6) & ~3 <= 1020, so T + H + 4 <= T + 1024
 
 int align_flags(const float *p, int64_t ld) {
@@ -336,8 +563,14 @@ int fir_direct_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B
     if (K == 0) return fill_zero_2d(y, ldy, B, N, stream);
     const TuneChoice tc = tune_lookup(K, N);
     const bool r8 = (tc.direct_r == 8) || N <= 2048;
+    // F32 mode on the matrix cores (strips of direct_mfma_strip 256-output
+    // tiles); F32_EXACT and longer filters take the LDS-DMA loader's rules.
+    if (tc.direct_loader == 3 && !exact && K <= kMfmaF32MaxK) {
+        const int strip = tc.direct_mfma_strip > 0 ? tc.direct_mfma_strip : (K <= 64 ? 8 : 32);
+        return launch_direct_mfma_k(x, ldx, b, K, B, N, y, ldy, strip, rev, stream);
+    }
     // LDS-DMA loader: forward rows, 16-byte aligned, N % 4 == 0, N < 2^29.
-    if (tc.direct_loader == 2 && K <= kPipeMaxK && !rev && N % 4 == 0 &&
+    if (tc.direct_loader >= 2 && K <= kPipeMaxK && !rev && N % 4 == 0 &&
         N < ((int64_t)1 << 29) && align_flags(x, ldx) && align_flags(y, ldy)) {
         const int st = tc.direct_seg_tiles;
         const int nb = tc.direct_dma_bufs;

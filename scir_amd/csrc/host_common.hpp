@@ -58,14 +58,16 @@ int guarded(const char *entry, F &&body) noexcept
 // Autotuned choices (defaults until a table is loaded or a search ran).
 struct TuneChoice {
     int direct_r = 8;         // outputs per lane in the direct kernel
-    int direct_loader = 2;    // 0: one tile per workgroup, 1: register prefetch, 2: LDS DMA
+    int direct_loader = 3;    // 0: one tile per workgroup, 1: register prefetch, 2: LDS DMA,
+                              // 3: F32 mode on FP32 MFMA (K <= 257; otherwise as 2)
+    int direct_mfma_strip = 0; // 256-output tiles per wave of loader 3 (0: by K)
     int direct_seg_tiles = 4; // tiles per workgroup of the pipelined variants
     int direct_dma_bufs = 2;  // LDS buffers of the DMA loader (tiles in flight + 1), 2..4
     int mfma_tiles = 2;       // 32x32 output tiles per wave, BF16x3 MFMA kernel
     int mfma_tiles_fp16 = 4;  // the same for the FP16 MFMA kernel
     int mfma_segs = 16;       // row segments per workgroup of the MFMA kernel
     bool mfma_segs_forced = false;  // set through scir_hip_tune_set: no grid-size cap
-    int auto_mfma_min_k = 56; // AUTO uses BF16x3 MFMA from this K up
+    int auto_mfma_min_k = 130; // AUTO uses BF16x3 MFMA from this K up (F32 on MFMA below)
     // AUTO may take the FFT path (normwise error) from auto_fft_min_k up,
     // when enabled with the tuning key auto.fft = 1 (off by default).
     bool auto_fft = false;

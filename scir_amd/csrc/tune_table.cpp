@@ -5,9 +5,11 @@
 //               "mfma_segs_per_item": 8, "mfma_wg_per_cu": 0},
 //    "entries": [{"k_lo": 1, "k_hi": 16, "n_lo": 0, "n_hi": 8192,
 //                 "direct_r": 8, "direct_loader": 1, "direct_seg_tiles": 2,
+//                 "direct_mfma_strip": 8,
 //                 "mfma_tiles": 2, "mfma_tiles_fp16": 4, "mfma_segs": 16}, ...]}
-// "auto_fft_min_k", "ragged" and "mfma_segs" are optional (tables written
-// before they existed); out-of-range values fall back to the defaults.
+// "auto_fft_min_k", "ragged", "mfma_segs" and "direct_mfma_strip" are
+// optional (tables written before they existed); out-of-range values fall
+// back to the defaults.
 #include <algorithm>
 #include <cerrno>
 #include <cstdlib>
@@ -87,12 +89,15 @@ int parse_tune_table(const std::string &s, const char *source, TuneTable *out)
         e.n_lo = v[2];
         e.n_hi = v[3];
         e.c.direct_r = v[4] == 16 ? 16 : 8;
-        e.c.direct_loader = clamp_int(v[5], 0, 2, 1);
+        e.c.direct_loader = clamp_int(v[5], 0, 3, 1);
         e.c.direct_seg_tiles = clamp_int(v[6], 1, 1 << 20, 4);
         e.c.mfma_tiles = (v[7] == 1 || v[7] == 4) ? (int)v[7] : 2;
         e.c.mfma_tiles_fp16 = (v[8] == 1 || v[8] == 2) ? (int)v[8] : 4;
         int64_t segs = 0;
         if (find_int(s, a, b, "mfma_segs", &segs)) e.c.mfma_segs = clamp_int(segs, 1, 1 << 20, 16);
+        int64_t strip = 0;
+        if (find_int(s, a, b, "direct_mfma_strip", &strip))
+            e.c.direct_mfma_strip = clamp_int(strip, 0, 4096, 0);
         t.entries.push_back(e);
         p = b;
     }
@@ -137,6 +142,7 @@ std::string tune_table_json(const TuneTable &t)
           << ", \"n_lo\": " << e.n_lo << ", \"n_hi\": " << e.n_hi
           << ", \"direct_r\": " << e.c.direct_r << ", \"direct_loader\": " << e.c.direct_loader
           << ", \"direct_seg_tiles\": " << e.c.direct_seg_tiles
+          << ", \"direct_mfma_strip\": " << e.c.direct_mfma_strip
           << ", \"mfma_tiles\": " << e.c.mfma_tiles
           << ", \"mfma_tiles_fp16\": " << e.c.mfma_tiles_fp16
           << ", \"mfma_segs\": " << e.c.mfma_segs << "}";

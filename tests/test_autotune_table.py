@@ -8,7 +8,8 @@ from scir_amd import _lib
 TABLE = """{"arch": "gfx950", "auto_mfma_min_k": 128,
  "entries": [
   {"k_lo": 1, "k_hi": 16, "n_lo": 0, "n_hi": 8192, "direct_r": 8, "direct_loader": 1, "direct_seg_tiles": 2, "mfma_tiles": 2, "mfma_tiles_fp16": 4},
-  {"k_lo": 17, "k_hi": 64, "n_lo": 8193, "n_hi": 4611686018427387904, "direct_r": 16, "direct_loader": 0, "direct_seg_tiles": 1, "mfma_tiles": 1, "mfma_tiles_fp16": 2, "mfma_segs": 4}]}
+  {"k_lo": 17, "k_hi": 64, "n_lo": 8193, "n_hi": 4611686018427387904, "direct_r": 16, "direct_loader": 0, "direct_seg_tiles": 1, "mfma_tiles": 1, "mfma_tiles_fp16": 2, "mfma_segs": 4},
+  {"k_lo": 65, "k_hi": 256, "n_lo": 0, "n_hi": 4611686018427387904, "direct_r": 8, "direct_loader": 3, "direct_seg_tiles": 4, "direct_mfma_strip": 32, "mfma_tiles": 2, "mfma_tiles_fp16": 4}]}
 """
 
 
@@ -36,7 +37,14 @@ def test_overrides_validated():
     assert L.scir_hip_tune_set(b"mfma.tiles", 3) == _lib.ERR_INVALID_ARGUMENT
     assert L.scir_hip_tune_set(b"mfma.segs", 0) == _lib.ERR_INVALID_ARGUMENT
     assert L.scir_hip_tune_set(b"direct.dma_bufs", 5) == _lib.ERR_INVALID_ARGUMENT
-    assert L.scir_hip_tune_set(b"direct.loader", 3) == _lib.ERR_INVALID_ARGUMENT
+    assert L.scir_hip_tune_set(b"direct.loader", 4) == _lib.ERR_INVALID_ARGUMENT
+    assert L.scir_hip_tune_set(b"direct.loader", 3) == _lib.OK
+    assert L.scir_hip_tune_set(b"direct.loader", -1) == _lib.OK
+    assert L.scir_hip_tune_set(b"direct.mfma_strip", 0) == _lib.ERR_INVALID_ARGUMENT
+    assert L.scir_hip_tune_set(b"direct.mfma_strip", 4097) == _lib.ERR_INVALID_ARGUMENT
+    assert b"direct.mfma_strip" in L.scir_hip_last_error()
+    assert L.scir_hip_tune_set(b"direct.mfma_strip", 16) == _lib.OK
+    assert L.scir_hip_tune_set(b"direct.mfma_strip", -1) == _lib.OK
     assert L.scir_hip_tune_set(b"direct.dma_bufs", 2) == _lib.OK
     assert L.scir_hip_tune_set(b"direct.dma_bufs", -1) == _lib.OK
     assert L.scir_hip_tune_set(b"mfma.segs", 8) == _lib.OK

@@ -33,20 +33,27 @@ def host_fir(hip, x, b, precision):
 
 
 @pytest.fixture(params=[(0, 16, 1, 3), (0, 8, 1, 3), (1, 16, 8, 3), (1, 8, 3, 3), (1, 8, 1, 3),
-                        (2, 8, 8, 3), (2, 8, 1, 3), (2, 8, 2, 2), (2, 8, 5, 4), (2, 16, 3, 3)],
-                ids=lambda p: f"loader{p[0]}-r{p[1]}-seg{p[2]}-bufs{p[3]}")
+                        (2, 8, 8, 3), (2, 8, 1, 3), (2, 8, 2, 2), (2, 8, 5, 4), (2, 16, 3, 3),
+                        (3, 8, 4, 2, 1), (3, 8, 4, 2, 3), (3, 8, 4, 2, 8), (3, 8, 1, 2, 64)],
+                ids=lambda p: f"loader{p[0]}-r{p[1]}-seg{p[2]}-bufs{p[3]}" +
+                              (f"-strip{p[4]}" if len(p) > 4 else ""))
 def variant(request, hip):
     """Each direct-kernel variant: one tile per workgroup or pipelined row
     segments of `seg` tiles (register prefetch, or LDS DMA with `bufs` tile
-    buffers), 8 or 16 outputs per lane."""
-    loader, r, seg, bufs = request.param
+    buffers), 8 or 16 outputs per lane; loader 3 runs F32 on FP32 MFMA in
+    strips of `strip` 256-output tiles per wave (K <= 257; F32_EXACT and
+    longer filters take loader 2's rules)."""
+    loader, r, seg, bufs = request.param[:4]
+    strip = request.param[4] if len(request.param) > 4 else -1
     L = hip.lib()
     hip.check(L.scir_hip_tune_set(b"direct.loader", loader))
     hip.check(L.scir_hip_tune_set(b"direct.r", r))
     hip.check(L.scir_hip_tune_set(b"direct.seg_tiles", seg))
     hip.check(L.scir_hip_tune_set(b"direct.dma_bufs", bufs))
+    hip.check(L.scir_hip_tune_set(b"direct.mfma_strip", strip))
     yield request.param
-    for key in (b"direct.loader", b"direct.r", b"direct.seg_tiles", b"direct.dma_bufs"):
+    for key in (b"direct.loader", b"direct.r", b"direct.seg_tiles", b"direct.dma_bufs",
+                b"direct.mfma_strip"):
         hip.check(L.scir_hip_tune_set(key, -1))
 
 

@@ -25,18 +25,23 @@ from test_gpu_direct import host_fir
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=[(0, 8, 1, 3), (0, 16, 1, 3), (1, 8, 3, 3), (2, 8, 4, 2), (2, 16, 3, 3)],
-                ids=lambda p: f"loader{p[0]}-r{p[1]}-seg{p[2]}-bufs{p[3]}")
+@pytest.fixture(params=[(0, 8, 1, 3), (0, 16, 1, 3), (1, 8, 3, 3), (2, 8, 4, 2), (2, 16, 3, 3),
+                        (3, 8, 4, 2, 1), (3, 8, 4, 2, 6)],
+                ids=lambda p: f"loader{p[0]}-r{p[1]}-seg{p[2]}-bufs{p[3]}" +
+                              (f"-strip{p[4]}" if len(p) > 4 else ""))
 def variant(request, hip):
     """Direct-kernel variants: one tile per workgroup, register prefetch and
-    LDS DMA, 8 or 16 outputs per lane."""
-    loader, r, seg, bufs = request.param
+    LDS DMA, 8 or 16 outputs per lane; F32 on FP32 MFMA (loader 3) in strips
+    of 1 and 6 tiles."""
+    loader, r, seg, bufs = request.param[:4]
+    strip = request.param[4] if len(request.param) > 4 else -1
     L = hip.lib()
     for key, v in ((b"direct.loader", loader), (b"direct.r", r), (b"direct.seg_tiles", seg),
-                   (b"direct.dma_bufs", bufs)):
+                   (b"direct.dma_bufs", bufs), (b"direct.mfma_strip", strip)):
         hip.check(L.scir_hip_tune_set(key, v))
     yield request.param
-    for key in (b"direct.loader", b"direct.r", b"direct.seg_tiles", b"direct.dma_bufs"):
+    for key in (b"direct.loader", b"direct.r", b"direct.seg_tiles", b"direct.dma_bufs",
+                b"direct.mfma_strip"):
         hip.check(L.scir_hip_tune_set(key, -1))
 
 

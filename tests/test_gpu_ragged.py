@@ -3,7 +3,8 @@
 Reference: per-signal FP64 oracle (oracle.lfilter_ragged_f64) within the
 (K_s+1)*2^-23*sum|b||x| bound; the f32_exact mode bit for bit against the
 reference loop per signal; the persistent launch against one dense launch
-per signal (same arithmetic: identical bits).  The MFMA precisions (bf16x3,
+per signal of the same arithmetic, the VALU direct form (loader 2; dense
+F32 calls with K <= 257 otherwise run on FP32 MFMA): identical bits.  The MFMA precisions (bf16x3,
 fp16) run a persistent Toeplitz contraction whose per-output arithmetic is
 that of the dense MFMA kernel, so they too equal the per-signal dense launches
 bit for bit; bf16x3 is held to relative L2 error <= 1e-6 against FP64."""
@@ -28,6 +29,19 @@ def make_case(seed, S, max_len, max_k, allow_empty=True):
     x = r.standard_normal(int(offs[-1])).astype(np.float32)
     b = r.standard_normal(max(int(boffs[-1]), 1)).astype(np.float32)
     return offs, boffs, x, b
+
+
+class valu_direct_form:
+    """Dense F32 calls on the VALU direct form (direct.loader 2), the
+    arithmetic of the ragged persistent F32 kernel, while in effect."""
+
+    def __enter__(self):
+        from scir_amd import _lib
+        _lib.check(_lib.lib().scir_hip_tune_set(b"direct.loader", 2))
+
+    def __exit__(self, *exc):
+        from scir_amd import _lib
+        _lib.check(_lib.lib().scir_hip_tune_set(b"direct.loader", -1))
 
 
 def run(offs, boffs, x, b, precision, per_signal=False):
@@ -78,8 +92,9 @@ def test_ragged_exact_bitwise(hip, oracle, seed):
 def test_persistent_equals_per_signal(hip):
     offs, boffs, x, b = make_case(7, 80, 12000, 900)
     for prec in ("f32", "f32_exact"):
-        np.testing.assert_array_equal(run(offs, boffs, x, b, prec),
-                                      run(offs, boffs, x, b, prec, per_signal=True))
+        with valu_direct_form():
+            y_sig = run(offs, boffs, x, b, prec, per_signal=True)
+        np.testing.assert_array_equal(run(offs, boffs, x, b, prec), y_sig)
 
 
 @pytest.mark.parametrize("precision", ["bf16x3", "fp16"])
@@ -203,7 +218,8 @@ def test_full_size_ragged_workload(hip, oracle):
     bt = torch.tensor(taps, device="cuda")
     plan = RaggedPlan(offs, boffs, "f32")
     y = plan.execute(x, bt)
-    y2 = plan.execute_per_signal(x, bt)
+    with valu_direct_form():
+        y2 = plan.execute_per_signal(x, bt)
     sabs = plan.execute(x.abs(), bt.abs())
     torch.cuda.synchronize()
     assert torch.equal(y, y2)
