@@ -369,7 +369,9 @@ SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
  *                     next segment's samples load while the current one computes
  *   "vjp.db_mfma"     1 (default): dL/db on FP32 MFMA; 0: the FP32 VALU kernel
  *   "auto.mfma_min_k" smallest K for which AUTO picks BF16x3 MFMA (default
- *                     130: F32 on FP32 MFMA is faster below)
+ *                     130: F32 on FP32 MFMA is faster below; ragged plans,
+ *                     whose F32 kernel is the VALU form: 56, by the batch's
+ *                     multiply-add weighted mean K)
  *   "auto.fft"        1: AUTO may pick the FFT32 path (normwise error) from
  *                     auto.fft_min_k up (x != y); 0 (default): never
  *   "auto.fft_min_k"  smallest K for which AUTO (with auto.fft) picks FFT32;

@@ -109,6 +109,7 @@ RaggedTune ragged_tune_lookup()
     if (g_over_rg_direct_kmacs >= 0) r.direct_min_item_kmacs = g_over_rg_direct_kmacs;
     if (g_over_rg_mfma_segs >= 0) r.mfma_segs_per_item = g_over_rg_mfma_segs;
     if (g_over_rg_mfma_wg >= 0) r.mfma_wg_per_cu = g_over_rg_mfma_wg;
+    if (g_over_auto_min_k >= 0) r.auto_mfma_min_k = g_over_auto_min_k;
     return r;
 }
 

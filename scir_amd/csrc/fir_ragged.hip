@@ -144,10 +144,12 @@ int scir_hip_ragged_plan_create(const int64_t *offs, const int64_t *b_offs, int6
         *plan = nullptr;
         SCIR_TRY(validate_csr(offs, b_offs, S));
         if (precision == SCIR_HIP_PREC_AUTO) {
-            // As for dense batches: BF16x3 MFMA when the batch's taps (mean K
-            // weighted by multiply-adds) reach the tuned cut-over and its LDS
-            // holds the longest filter; the FP32 direct form otherwise.  Both
-            // keep the elementwise FP32 bound.
+            // BF16x3 MFMA when the batch's taps (mean K weighted by
+            // multiply-adds) reach the ragged cut-over and its LDS holds the
+            // longest filter; the FP32 direct form otherwise.  Both keep the
+            // elementwise FP32 bound.  The ragged F32 kernel is the VALU form,
+            // so its cut-over is lower than the dense one (whose F32 runs on
+            // FP32 MFMA up to K = 257).
             int kmean = 0, kmax_all = 0;
             int64_t nmean = 0;
             ragged_batch_stats(offs, b_offs, S, &kmean, &nmean, &kmax_all);
@@ -156,9 +158,9 @@ int scir_hip_ragged_plan_create(const int64_t *offs, const int64_t *b_offs, int6
                               fir_mfma_ragged_lds(SCIR_HIP_PREC_BF16X3, kmax_all, &lds_need) ==
                                   SCIR_HIP_OK;
             clear_error();
-            const TuneChoice tc = tune_lookup(std::max(kmean, 1), nmean);
-            precision = (fits && kmean >= tc.auto_mfma_min_k) ? SCIR_HIP_PREC_BF16X3
-                                                               : SCIR_HIP_PREC_F32;
+            precision = (fits && kmean >= ragged_tune_lookup().auto_mfma_min_k)
+                            ? SCIR_HIP_PREC_BF16X3
+                            : SCIR_HIP_PREC_F32;
         }
         RaggedMode mode = kDirect;
         SCIR_TRY(ragged_mode(precision, &mode));

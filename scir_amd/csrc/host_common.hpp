@@ -82,6 +82,7 @@ struct RaggedTune {
     int direct_min_item_kmacs = 512;  // multiply-adds per direct work item, at least (x1024)
     int mfma_segs_per_item = 8;       // segments per MFMA work item, at most
     int mfma_wg_per_cu = 0;           // persistent workgroups per CU, MFMA (0: occupancy)
+    int auto_mfma_min_k = 56;         // AUTO plans take BF16x3 from this weighted mean K up
 };
 
 // One row of the tuning table: a (K, N) bucket and its launch shapes.
