@@ -373,7 +373,11 @@ __global__ __launch_bounds__(kThreads) void fir_direct_mfma_kernel(
     constexpr int D = 3;
     constexpr int RING = D + 2;
     f32x4 q[RING];
-    q[0] = load4(xrow, (t0 - 1) * kMfmaTile + off, N, rv);  // zeros before the row
+    // The first tile's history: only lanes j >= 16 - (NG-1) of the previous
+    // tile are read by the DPP rotates (J <= NG-1), so only they load it (at
+    // K = 33 the last 32 samples, not 256: the strip's one extra read).
+    // Zeros before the row start.
+    q[0] = li >= 16 - (NG - 1) ? load4(xrow, (t0 - 1) * kMfmaTile + off, N, rv) : zero;
 #pragma unroll
     for (int d = 0; d < D; ++d)
         q[d + 1] = d < nt ? load4(xrow, (t0 + d) * kMfmaTile + off, N, rv) : zero;
