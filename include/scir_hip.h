@@ -361,7 +361,7 @@ SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
  *                     F32 on FP32 MFMA for K <= 257 (F32_EXACT and longer
  *                     filters as 2)
  *   "direct.mfma_strip" 256-output tiles per wave of loader 3 (1 .. 4096;
- *                     default 8 for K <= 64, else 32)
+ *                     default 4 for K <= 8, 8 to K = 40, 16 to K = 56, else 32)
  *   "direct.seg_tiles" tiles per workgroup of the pipelined direct kernel
  *   "direct.dma_bufs" LDS tile buffers of loader 2 (2, 3 or 4: 1 to 3 tiles in flight)
  *   "mfma.tiles"      32x32 output tiles per wave of the MFMA kernel (1, 2, 4)
