@@ -263,45 +263,52 @@ struct Contraction {
     }
 };
 
-// Samples of LDS unit u (8 samples from xb + 8u of the row, logical order;
-// zero outside [0, N) and for u >= units).  In rev mode logical sample i is
-// physical sample N-1-i.
+// Samples of LDS unit u: 8 samples from xb + 8u of the row (logical order),
+// zero outside [0, N) and for u >= units.  With REV logical sample i is
+// physical sample N-1-i, and r holds the unit in physical order (slot p =
+// logical sample g0 + 7 - p), which unit_sample reads back to front.  Every
+// load lands in its slot of r with no register move after it: a move (or a
+// select) would wait for the load, and the prefetch of the next segment must
+// stay in flight through the contraction.
+template <bool REV>
 __device__ __forceinline__ void load_unit(const float *__restrict__ xrow, int64_t N, int64_t xb,
-                                          int u, int units, int rev, f32x4 (&r)[2])
+                                          int u, int units, f32x4 (&r)[2])
 {
     const int64_t g0 = xb + 8 * (int64_t)u;
     if (u >= units) {
         r[0] = r[1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     } else if (g0 >= 0 && g0 + 7 < N) {
-        if (!rev) {
-            r[0] = __builtin_nontemporal_load(reinterpret_cast<const f32x4u *>(xrow + g0));
-            r[1] = __builtin_nontemporal_load(reinterpret_cast<const f32x4u *>(xrow + g0 + 4));
-        } else {  // logical g0..g0+7 == physical N-1-g0 .. N-8-g0
-            const f32x4 a =
-                __builtin_nontemporal_load(reinterpret_cast<const f32x4u *>(xrow + (N - 4 - g0)));
-            const f32x4 c =
-                __builtin_nontemporal_load(reinterpret_cast<const f32x4u *>(xrow + (N - 8 - g0)));
-            r[0] = f32x4{a.w, a.z, a.y, a.x};
-            r[1] = f32x4{c.w, c.z, c.y, c.x};
-        }
+        const float *p = REV ? xrow + (N - 8 - g0) : xrow + g0;
+        r[0] = __builtin_nontemporal_load(reinterpret_cast<const f32x4u *>(p));
+        r[1] = __builtin_nontemporal_load(reinterpret_cast<const f32x4u *>(p + 4));
     } else {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int64_t i = g0 + e;
-            r[e >> 2][e & 3] = (i >= 0 && i < N) ? xrow[rev ? N - 1 - i : i] : 0.0f;
+            const int slot = REV ? 7 - e : e;
+            r[slot >> 2][slot & 3] = 0.0f;
+            if (i >= 0 && i < N) r[slot >> 2][slot & 3] = xrow[REV ? N - 1 - i : i];
         }
     }
 }
 
+// Logical sample e (0..7) of a unit loaded by load_unit<REV>.
+template <bool REV>
+__device__ __forceinline__ float unit_sample(const f32x4 (&r)[2], int e)
+{
+    const int slot = REV ? 7 - e : e;
+    return r[slot >> 2][slot & 3];
+}
+
 // Split the 8 samples of unit u into the S split arrays of the LDS.
-template <int S, int TW>
+template <int S, int TW, bool REV>
 __device__ __forceinline__ void store_unit(unsigned char *lds, int split_stride, int u,
                                            const f32x4 (&r)[2])
 {
     unsigned short t[S][8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-        const float v = r[e >> 2][e & 3];
+        const float v = unit_sample<REV>(r, e);
         if (S == 3)
             split3(v, t[0][e], t[S > 1 ? 1 : 0][e], t[S > 2 ? 2 : 0][e]);
         else
@@ -353,34 +360,45 @@ __device__ __forceinline__ void build_tap_table(unsigned char *lds, int tab_offs
             }
         }
     }
+    // Every tap load above is consumed, but the compiler cannot tell after the
+    // early exit; without this vmcnt(0) it would carry them as pending into
+    // the segment loop and wait for each segment's prefetch before the
+    // contraction.  (The first segment's samples, loaded before the table,
+    // are staged right after this anyway.)
+    __builtin_amdgcn_s_waitcnt(0x0f70);
 }
 
 // Stage LDS units 0 .. units-1 of a segment (unit 0 = sample xb of the row):
 // the first PF units of each lane come from `raw` (loaded ahead), the rest
 // are loaded here.
-template <int S, int TW, int PF>
+template <int S, int TW, int PF, bool REV>
 __device__ __forceinline__ void stage_segment(unsigned char *lds, int split_stride,
                                               const f32x4 (&raw)[PF][2],
                                               const float *__restrict__ xrow, int64_t N,
-                                              int64_t xb, int units, int rev, int tid)
+                                              int64_t xb, int units, int tid)
 {
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
         const int u = tid + j * kThreads;
-        if (u < units) store_unit<S, TW>(lds, split_stride, u, raw[j]);
+        if (u < units) store_unit<S, TW, REV>(lds, split_stride, u, raw[j]);
     }
+    if (units <= PF * kThreads) return;  // K <= 2049
     for (int u0 = tid + PF * kThreads; u0 < units; u0 += kRestBatch * kThreads) {
         f32x4 rest[kRestBatch][2];
 #pragma unroll
         for (int j = 0; j < kRestBatch; ++j)
-            load_unit(xrow, N, xb, u0 + j * kThreads, units, rev, rest[j]);
+            load_unit<REV>(xrow, N, xb, u0 + j * kThreads, units, rest[j]);
 #pragma unroll
         for (int j = 0; j < kRestBatch; ++j) {
             const int u = u0 + j * kThreads;
             if (u >= units) break;
-            store_unit<S, TW>(lds, split_stride, u, rest[j]);
+            store_unit<S, TW, REV>(lds, split_stride, u, rest[j]);
         }
     }
+    // Every load above has been consumed (a unit past `units` is zeros, not
+    // loaded), but the compiler cannot tell, and would wait for the next
+    // segment's prefetch before the contraction instead: vmcnt(0) here.
+    __builtin_amdgcn_s_waitcnt(0x0f70);
 }
 
 // Contraction of one staged segment by wave `wave`; lane (i, g) = (lane % 32,
@@ -414,10 +432,10 @@ __device__ __forceinline__ void contract_segment(Contraction<S, TW> &ct, const u
 // Epilogue: tile t, lane (i, g) holds block P0 + wave*32*TW + t + TW*i,
 // samples 8q + 4g + j; outputs past N are dropped.  A non-finite output is
 // recomputed from the reference loop's terms (x row xrow, taps b[0, K)).
-template <int S, int TW>
+template <int S, int TW, bool REV>
 __device__ __forceinline__ void store_segment(const Contraction<S, TW> &ct,
                                               float *__restrict__ yrow, int64_t N, int64_t P0,
-                                              int wave, int i, int g, int rev,
+                                              int wave, int i, int g,
                                               const float *__restrict__ xrow,
                                               const float *__restrict__ b, int K)
 {
@@ -437,17 +455,17 @@ __device__ __forceinline__ void store_segment(const Contraction<S, TW> &ct,
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                     if (n + j < N && !__builtin_isfinite(v[j]))
-                        v[j] = direct_output<false>(xrow, N, rev != 0, b, K, n + j);
+                        v[j] = direct_output<false>(xrow, N, REV, b, K, n + j);
             }
             if (n + 3 < N) {
-                if (!rev)
+                if (!REV)
                     *reinterpret_cast<f32x4u *>(yrow + n) = v;
                 else
                     *reinterpret_cast<f32x4u *>(yrow + (N - 4 - n)) = f32x4{v.w, v.z, v.y, v.x};
             } else {
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    if (n + j < N) yrow[rev ? N - 1 - (n + j) : n + j] = v[j];
+                    if (n + j < N) yrow[REV ? N - 1 - (n + j) : n + j] = v[j];
             }
         }
     }
@@ -461,12 +479,12 @@ __device__ __forceinline__ void store_segment(const Contraction<S, TW> &ct,
 // they are in flight while the waves run the previous segment's MFMAs.
 // Units past PF*256 (only when 2*m_hi > 256) are loaded when the segment is
 // staged.
-template <int S, int TW>
+template <int S, int TW, bool REV>
 __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
     const float *__restrict__ x, int64_t ldx, int64_t N, float *__restrict__ y, int64_t ldy,
     const float *__restrict__ b, int K, int msteps, int m_hi, int64_t row0, int segs_per_row,
     int64_t nsegs, int segs_per_wg, int64_t nwg, int split_stride, int tab_offset,
-    int copy_bytes, int rev)
+    int copy_bytes)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     constexpr int SEG_BLOCKS = kWaves * TW * 32;
@@ -491,7 +509,7 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
         seg_base(sg, xrow, xb, P0, row);
 #pragma unroll
         for (int j = 0; j < PF; ++j)
-            load_unit(xrow, N, xb, tid + j * kThreads, units, rev, raw[j]);
+            load_unit<REV>(xrow, N, xb, tid + j * kThreads, units, raw[j]);
     };
     prefetch(seg_first, (int)threadIdx.x);
 
@@ -512,7 +530,7 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
 
         // ---- stage: x[xb .. xb + 8*units) -> S split arrays in LDS ----
         if (sg != seg_first) __syncthreads();  // the previous segment's MFMAs are done
-        stage_segment<S, TW, PF>(lds, split_stride, raw, xrow, N, xb, units, rev, tid);
+        stage_segment<S, TW, PF, REV>(lds, split_stride, raw, xrow, N, xb, units, tid);
         __syncthreads();
         if (sg + 1 < seg_end) prefetch(sg + 1, tid);
 
@@ -520,7 +538,7 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
         Contraction<S, TW> ct;
         contract_segment<S, TW>(ct, lds, split_stride, tab_offset, copy_bytes, msteps, m_hi, wave,
                                 i, g);
-        store_segment<S, TW>(ct, y + row * ldy, N, P0, wave, i, g, rev, xrow, b, K);
+        store_segment<S, TW, REV>(ct, y + row * ldy, N, P0, wave, i, g, xrow, b, K);
     }
 }
 
@@ -560,7 +578,7 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_ragged_kernel(
             const int64_t xb = 32 * sg * SEG_BLOCKS - 16 * (int64_t)m_hi;
 #pragma unroll
             for (int j = 0; j < PF; ++j)
-                load_unit(xrow, w.len, xb, tid + j * kThreads, units, 0, raw[j]);
+                load_unit<false>(xrow, w.len, xb, tid + j * kThreads, units, raw[j]);
         };
         prefetch(w.seg0, (int)threadIdx.x);
         if (w.b0 != tab_b0 || w.K != tab_K) {
@@ -576,13 +594,13 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_ragged_kernel(
             const int64_t P0 = sg * SEG_BLOCKS;
             const int64_t xb = 32 * P0 - 16 * (int64_t)m_hi;
             if (sg != w.seg0) __syncthreads();  // the previous segment's MFMAs are done
-            stage_segment<S, TW, PF>(lds, split_stride, raw, xrow, w.len, xb, units, 0, tid);
+            stage_segment<S, TW, PF, false>(lds, split_stride, raw, xrow, w.len, xb, units, tid);
             __syncthreads();
             if (sg + 1 < seg_end) prefetch(sg + 1, tid);
             Contraction<S, TW> ct;
             contract_segment<S, TW>(ct, lds, split_stride, tab_offset, copy_bytes, msteps, m_hi,
                                     wave, i, g);
-            store_segment<S, TW>(ct, yrow, w.len, P0, wave, i, g, 0, xrow, b + w.b0,
+            store_segment<S, TW, false>(ct, yrow, w.len, P0, wave, i, g, xrow, b + w.b0,
                                  w.K);
         }
     }
@@ -599,9 +617,9 @@ size_t mfma_lds_bytes(int K)
     return (size_t)S * unit_bytes(units, TW) + (size_t)2 * S * tab_copy_bytes(m_hi, TW);
 }
 
-template <int S, int TW>
-int launch_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
-                float *y, int64_t ldy, bool rev, int segs_per_wg, hipStream_t stream)
+template <int S, int TW, bool REV>
+int launch_mfma_dir(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
+                    float *y, int64_t ldy, int segs_per_wg, hipStream_t stream)
 {
     constexpr int SEG_BLOCKS = kWaves * TW * 32;
     const int m_hi = mfma_m_hi(K), msteps = mfma_steps(K);
@@ -611,7 +629,7 @@ int launch_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, i
     const int copy_bytes = tab_copy_bytes(m_hi, TW);
     const size_t lds = mfma_lds_bytes<S, TW>(K);
     SCIR_TRY(
-        allow_dynamic_lds(reinterpret_cast<const void *>(&fir_mfma_kernel<S, TW>), kMaxLds));
+        allow_dynamic_lds(reinterpret_cast<const void *>(&fir_mfma_kernel<S, TW, REV>), kMaxLds));
     const int64_t seg_len = 32 * (int64_t)SEG_BLOCKS;
     const int64_t segs = (N + seg_len - 1) / seg_len;
     if (segs > 0x7fffffff)
@@ -621,12 +639,22 @@ int launch_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, i
         const int64_t rows = std::min(rows_per_launch, B - r0);
         const int64_t nsegs = rows * segs;
         const int64_t nwg = (nsegs + segs_per_wg - 1) / segs_per_wg;
-        hipLaunchKernelGGL((fir_mfma_kernel<S, TW>), dim3((unsigned)nwg), dim3(kThreads), lds,
-                           stream, x, ldx, N, y, ldy, b, K, msteps, m_hi, r0, (int)segs, nsegs,
-                           segs_per_wg, nwg, split_stride, tab_offset, copy_bytes, (int)rev);
+        hipLaunchKernelGGL((fir_mfma_kernel<S, TW, REV>), dim3((unsigned)nwg), dim3(kThreads),
+                           lds, stream, x, ldx, N, y, ldy, b, K, msteps, m_hi, r0, (int)segs,
+                           nsegs, segs_per_wg, nwg, split_stride, tab_offset, copy_bytes);
         SCIR_TRY(check_launch("fir_mfma_kernel"));
     }
     return SCIR_HIP_OK;
+}
+
+// The direction is a template argument: a run-time flag puts a join after
+// every prefetch load, at which the compiler waits for the loads in flight.
+template <int S, int TW>
+int launch_mfma(const float *x, int64_t ldx, const float *b, int K, int64_t B, int64_t N,
+                float *y, int64_t ldy, bool rev, int segs_per_wg, hipStream_t stream)
+{
+    return rev ? launch_mfma_dir<S, TW, true>(x, ldx, b, K, B, N, y, ldy, segs_per_wg, stream)
+               : launch_mfma_dir<S, TW, false>(x, ldx, b, K, B, N, y, ldy, segs_per_wg, stream);
 }
 
 template <int S>
