@@ -535,10 +535,16 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
         if (sg + 1 < seg_end) prefetch(sg + 1, tid);
 
         // ---- contraction and epilogue ----
-        Contraction<S, TW> ct;
-        contract_segment<S, TW>(ct, lds, split_stride, tab_offset, copy_bytes, msteps, m_hi, wave,
-                                i, g);
-        store_segment<S, TW, REV>(ct, y + row * ldy, N, P0, wave, i, g, xrow, b, K);
+        // Wave w's tiles cover outputs 32*(P0 + 32*TW*w) .. +1024*TW-1 of the
+        // row: a wave past the row end has no output and skips both (the
+        // contraction has no barrier), leaving the SIMD's matrix pipe to the
+        // other wave on it.
+        if (32 * (P0 + (int64_t)__builtin_amdgcn_readfirstlane(wave) * 32 * TW) < N) {
+            Contraction<S, TW> ct;
+            contract_segment<S, TW>(ct, lds, split_stride, tab_offset, copy_bytes, msteps, m_hi,
+                                    wave, i, g);
+            store_segment<S, TW, REV>(ct, y + row * ldy, N, P0, wave, i, g, xrow, b, K);
+        }
     }
 }
 
@@ -597,11 +603,15 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_ragged_kernel(
             stage_segment<S, TW, PF, false>(lds, split_stride, raw, xrow, w.len, xb, units, tid);
             __syncthreads();
             if (sg + 1 < seg_end) prefetch(sg + 1, tid);
-            Contraction<S, TW> ct;
-            contract_segment<S, TW>(ct, lds, split_stride, tab_offset, copy_bytes, msteps, m_hi,
-                                    wave, i, g);
-            store_segment<S, TW, false>(ct, yrow, w.len, P0, wave, i, g, xrow, b + w.b0,
-                                 w.K);
+            // A wave whose outputs all lie past the signal's end skips its
+            // contraction and epilogue (as in the dense kernel).
+            if (32 * (P0 + (int64_t)__builtin_amdgcn_readfirstlane(wave) * 32 * TW) < w.len) {
+                Contraction<S, TW> ct;
+                contract_segment<S, TW>(ct, lds, split_stride, tab_offset, copy_bytes, msteps,
+                                        m_hi, wave, i, g);
+                store_segment<S, TW, false>(ct, yrow, w.len, P0, wave, i, g, xrow, b + w.b0,
+                                            w.K);
+            }
         }
     }
 }
