@@ -553,8 +553,10 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
 // counter until the queue is drained; every workgroup leaves the loop on its
 // first fetch past the end.  Each segment is staged, contracted and stored as
 // in the dense kernel, with the item's own taps and length; the tap tables are
-// rebuilt only when the item's taps differ from the ones in LDS.  The item
-// index is broadcast through the 16 bytes of LDS at `slot_offset`.
+// rebuilt only when the item's taps differ from the ones in LDS.  Items are
+// claimed one ahead and broadcast through the 16 bytes of LDS at
+// `slot_offset`; the next item's first segment loads during the current
+// item's last contraction.
 template <int S, int TW>
 __global__ __launch_bounds__(kThreads) void fir_mfma_ragged_kernel(
     const float *__restrict__ x, const float *__restrict__ b, float *__restrict__ y,
@@ -564,29 +566,45 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_ragged_kernel(
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     constexpr int SEG_BLOCKS = kWaves * TW * 32;
     constexpr int PF = 2 * TW + 1;
+    // Two item slots, used in turn: the index claimed at the start of item n
+    // is read by every lane right after that item's first barrier, and the
+    // slot is written again two items later, after the next item's barrier.
     int *s_item = reinterpret_cast<int *>(lds + slot_offset);
     int64_t tab_b0 = -1;
     int tab_K = -1;
     f32x4 raw[PF][2];
-    for (;;) {
-        if (threadIdx.x == 0) *s_item = (int)atomicAdd(counter, 1u);
-        __syncthreads();  // every wave is also done with the previous item's LDS
-        const int it = *s_item;
-        __syncthreads();  // every lane has read the slot before it is written again
-        if (it >= n_items) break;
-        const MfmaRaggedItem w = items[it];
+    // The first segment of item q, into raw (also for the next item, during
+    // the current item's last contraction).
+    auto prefetch = [&](const MfmaRaggedItem &q, int64_t sg, int tid) {
+        const int mh = mfma_m_hi(q.K);
+        const int64_t xb = 32 * sg * SEG_BLOCKS - 16 * (int64_t)mh;
+#pragma unroll
+        for (int j = 0; j < PF; ++j)
+            load_unit<false>(x + q.x0, q.len, xb, tid + j * kThreads, SEG_BLOCKS * 4 + 2 * mh,
+                             raw[j]);
+    };
+    if (threadIdx.x == 0) s_item[0] = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    int nxt = s_item[0];
+    MfmaRaggedItem wn{};
+    if (nxt < n_items) {
+        wn = items[nxt];
+        prefetch(wn, wn.seg0, (int)threadIdx.x);
+    }
+    for (int parity = 1;; parity ^= 1) {
+        const int it = nxt;
+        if (it >= n_items) break;  // uniform: read from LDS after a barrier
+        const MfmaRaggedItem w = wn;
+        // Claim the item after this one now, so its first segment can load
+        // during this item's last contraction.
+        if (threadIdx.x == 0) s_item[parity] = (int)atomicAdd(counter, 1u);
+        __syncthreads();  // every wave is done with the previous item's LDS; slot written
+        nxt = s_item[parity];
         const int m_hi = mfma_m_hi(w.K), msteps = m_hi + 2;
         const int units = SEG_BLOCKS * 4 + 2 * m_hi;
         const float *__restrict__ xrow = x + w.x0;
         float *__restrict__ yrow = y + w.x0;
         const int64_t seg_end = (int64_t)w.seg0 + w.nseg;
-        auto prefetch = [&](int64_t sg, int tid) {
-            const int64_t xb = 32 * sg * SEG_BLOCKS - 16 * (int64_t)m_hi;
-#pragma unroll
-            for (int j = 0; j < PF; ++j)
-                load_unit<false>(xrow, w.len, xb, tid + j * kThreads, units, raw[j]);
-        };
-        prefetch(w.seg0, (int)threadIdx.x);
         if (w.b0 != tab_b0 || w.K != tab_K) {
             build_tap_table<S, TW>(lds, tab_offset, copy_bytes, b + w.b0, w.K, m_hi);
             tab_b0 = w.b0;
@@ -602,7 +620,12 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_ragged_kernel(
             if (sg != w.seg0) __syncthreads();  // the previous segment's MFMAs are done
             stage_segment<S, TW, PF, false>(lds, split_stride, raw, xrow, w.len, xb, units, tid);
             __syncthreads();
-            if (sg + 1 < seg_end) prefetch(sg + 1, tid);
+            if (sg + 1 < seg_end) {
+                prefetch(w, sg + 1, tid);
+            } else if (nxt < n_items) {
+                wn = items[nxt];
+                prefetch(wn, wn.seg0, tid);
+            }
             // A wave whose outputs all lie past the signal's end skips its
             // contraction and epilogue (as in the dense kernel).
             if (32 * (P0 + (int64_t)__builtin_amdgcn_readfirstlane(wave) * 32 * TW) < w.len) {
