@@ -233,3 +233,35 @@ def test_unaligned_rows_equal_aligned(hip, precision, ox, oy):
     assert torch.equal(ya, yu), precision
     yb = ybase.cpu().numpy()
     assert np.isnan(yb[:, :oy]).all() and np.isnan(yb[:, oy + N:]).all()
+
+
+@pytest.mark.parametrize("strip", [1, 3, 8])
+@pytest.mark.parametrize("K", [16, 17, 240, 241, 256, 257])
+def test_f32_mfma_boundaries_both_directions(hip, oracle, strip, K):
+    """F32 on FP32 MFMA (K <= 257) where the Toeplitz block gains a column
+    group (NG = ceil((K-1)/16) + 1 steps up after K = 16 and K = 240; K = 257
+    is the last tap count whose history is one tile back), in strips of 1, 3
+    and 8 tiles, on rows that end inside a tile, on one and past a strip:
+    forward, and back to front as dL/dx uses it (dx[m] = sum_k b[k] g[m+k]),
+    both within (K+1) 2^-23 sum|b||x| of FP64."""
+    import torch
+    from scir_amd import device
+    L = hip.lib()
+    hip.check(L.scir_hip_tune_set(b"direct.mfma_strip", strip))
+    try:
+        for N in (3, 255, 256, 257, 2049, 6001):
+            print(f"f32 mfma K={K} strip={strip} N={N}", flush=True)
+            r = rng(K * 31 + N + strip)
+            x = r.standard_normal((3, N)).astype(np.float32)
+            x[1, ::41] *= 1000.0
+            b = r.standard_normal(K).astype(np.float32)
+            xt, bt = torch.tensor(x, device="cuda"), torch.tensor(b, device="cuda")
+            y = device.fir(xt, bt, "f32").cpu().numpy()
+            assert np.all(np.abs(y - oracle.lfilter_f64(x, b)) <= oracle.fir_error_bound(x, b)), N
+            dx = device.vjp_dx(xt, bt, "f32").cpu().numpy()
+            xr = np.ascontiguousarray(x[:, ::-1])
+            dx64 = oracle.lfilter_f64(xr, b)[:, ::-1]
+            bound = oracle.fir_error_bound(xr, b)[:, ::-1]
+            assert np.all(np.abs(dx - dx64) <= bound), f"dL/dx N={N}"
+    finally:
+        hip.check(L.scir_hip_tune_set(b"direct.mfma_strip", -1))
