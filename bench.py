@@ -52,7 +52,7 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0   # dense bf16/fp16 MFMA, MI355X
 FP32_PEAK_TFLOPS = 157.3
-F32_MFMA_MAX_K = 257  # the F32 mode's FP32-MFMA kernel takes K up to here
+F32_MFMA_MAX_K = 513  # the F32 mode's FP32-MFMA kernel takes K up to here
 
 METRIC = "FIR Gsamples/s & %roofline: bf16x3 MFMA vs FP32 direct at rel L2 err<=1e-6"
 
@@ -302,7 +302,7 @@ def roofline_for(wl, name, prec, avg_launch_s):
                     ceiling_note=f"{prods} MFMA products x {mfma_steps(wl['K'])} half-steps of "
                                  "32x32x16 per 32x32 output tile at 2.5 PFLOP/s")
     flops = 2.0 * wl["K"] * samples
-    # F32 up to K = 257 runs on the FP32 matrix cores (a 16 x 16(NG) Toeplitz
+    # F32 up to K = 513 runs on the FP32 matrix cores (a 16 x 16(NG) Toeplitz
     # block per 256 outputs, NG = ceil((K-1)/16) + 1); FP32 MFMA and FP32
     # vector share the 157.3 TFLOP/s peak on MI355X.
     f32_mfma = prec == "f32" and wl["K"] <= F32_MFMA_MAX_K

@@ -72,7 +72,7 @@ typedef enum scir_hip_status {
 /* ---- arithmetic selection --------------------------------------------- */
 typedef enum scir_hip_precision {
     /* FP32 arithmetic, any summation order; error within
-     * (K+1) * 2^-23 * sum_k |b_k||x_{n-k}| of the exact sum.  K <= 257 runs
+     * (K+1) * 2^-23 * sum_k |b_k||x_{n-k}| of the exact sum.  K <= 513 runs
      * as an FP32 Toeplitz contraction on the matrix cores
      * (v_mfma_f32_16x16x4_f32, twice the non-packed VALU FMA rate), longer
      * filters as the direct form with fused multiply-adds on the VALU.  Both
@@ -358,10 +358,11 @@ SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
  *                     (register prefetch), 2 = pipelined row segments copied
  *                     into LDS by DMA two tiles ahead (forward, 16-byte aligned
  *                     rows with N % 4 == 0; other calls use 1), 3 (default) =
- *                     F32 on FP32 MFMA for K <= 257 (F32_EXACT and longer
+ *                     F32 on FP32 MFMA for K <= 513 (F32_EXACT and longer
  *                     filters as 2)
  *   "direct.mfma_strip" 256-output tiles per wave of loader 3 (1 .. 4096;
- *                     default 4 for K <= 8, 8 to K = 40, 16 to K = 56, else 32)
+ *                     default 4 for K <= 8, 8 to K = 40, 16 to K = 56, 32 to
+ *                     K = 257, else 64)
  *   "direct.seg_tiles" tiles per workgroup of the pipelined direct kernel
  *   "direct.dma_bufs" LDS tile buffers of loader 2 (2, 3 or 4: 1 to 3 tiles in flight)
  *   "mfma.tiles"      32x32 output tiles per wave of the MFMA kernel (1, 2, 4)

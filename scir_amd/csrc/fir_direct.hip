@@ -244,7 +244,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_dma_kernel(
     }
 }
 
-// ---- F32 mode on the matrix cores (direct.loader = 3, K <= kMfmaF32MaxK) ----
+// ---- F32 mode on the matrix cores (direct.loader = 3, K <= kMfmaF32MaxK = 513) ----
 //
 // v_mfma_f32_16x16x4_f32 issues 1024 FP32 multiply-adds per 32 cycles,
 // twice the rate of non-packed FP32 FMAs on the VALU.  A tile of 256
@@ -259,7 +259,8 @@ __global__ __launch_bounds__(kThreads) void fir_direct_dma_kernel(
 // (0, v) in component v.  Step (J, v) takes component v of lane (g, j - J):
 // a DPP row shift by J within the lane's row of 16, lanes j < J reading the
 // previous tile's registers (DPP row rotate); J = 16 is the previous tile
-// itself.  Every (tap, output) pair of the filter occurs once; the rest of
+// itself, J = 17 .. 31 the same window one tile further back, J = 32 the
+// tile before the previous one.  Every (tap, output) pair of the filter occurs once; the rest of
 // the 16 x 16NG block multiplies zero taps.  D comes back in lane (g, j) as
 // y[s + 16j + 4g .. +3]: one 16-byte store.
 //
@@ -273,45 +274,56 @@ __global__ __launch_bounds__(kThreads) void fir_direct_dma_kernel(
 // non-finite tap times a zero before the row start) is recomputed from the
 // reference loop's terms, so non-finite outputs sit where the reference
 // loop's do.
-constexpr int kMfmaF32MaxK = 257;  // NG <= 17: history from the previous tile only
+constexpr int kMfmaF32MaxK = 513;  // NG <= 33: history from the two previous tiles
 constexpr int kMfmaTile = 256;
 
-// Sample x[s + 16(j - J) + 4g + v] for lane (g, j), from component v of the
-// current tile (cur) and the previous one (prev).
+// Lane j of a row of 16 reading lane j - J of `hi` (j >= J) or lane
+// j - J + 16 of `lo` (j < J), 1 <= J <= 15.
 template <int J>
-__device__ __forceinline__ float history_sample(float cur, float prev)
+__device__ __forceinline__ float row_window(float hi, float lo)
 {
-    if constexpr (J == 0) {
+    // row_ror:J -- lane j reads lane (j - J) mod 16 of lo; then row_shr:J --
+    // lanes j >= J read lane j - J of hi, the others keep the rotated lo (no
+    // source lane: the write is skipped).
+    const int p = __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, lo), 0x120 + J, 0xF, 0xF, false);
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(p, __builtin_bit_cast(int, hi), 0x110 + J, 0xF, 0xF,
+                                           false));
+}
+
+// Sample x[s + 16(j - J) + 4g + v] for lane (g, j), from component v of the
+// current tile (cur) and the two before it (prev, prev2).
+template <int J>
+__device__ __forceinline__ float history_sample(float cur, float prev, float prev2)
+{
+    if constexpr (J == 0)
         return cur;
-    } else if constexpr (J == 16) {
+    else if constexpr (J < 16)
+        return row_window<J>(cur, prev);
+    else if constexpr (J == 16)
         return prev;
-    } else {
-        // row_ror:J -- lane j reads lane (j - J) mod 16 of prev; then
-        // row_shr:J -- lanes j >= J read lane j - J of cur, the others keep
-        // the rotated prev (no source lane: the write is skipped).
-        const int p = __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, prev), 0x120 + J, 0xF,
-                                               0xF, false);
-        return __builtin_bit_cast(
-            float, __builtin_amdgcn_update_dpp(p, __builtin_bit_cast(int, cur), 0x110 + J, 0xF,
-                                               0xF, false));
-    }
+    else if constexpr (J < 32)
+        return row_window<J - 16>(prev, prev2);
+    else
+        return prev2;
 }
 
 template <int J>
 __device__ __forceinline__ void toeplitz_step(f32x4 &acc, const float (&a)[4], const f32x4 &cur,
-                                              const f32x4 &prev)
+                                              const f32x4 &prev, const f32x4 &prev2)
 {
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], history_sample<J>(cur.x, prev.x), acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], history_sample<J>(cur.y, prev.y), acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], history_sample<J>(cur.z, prev.z), acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], history_sample<J>(cur.w, prev.w), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], history_sample<J>(cur.x, prev.x, prev2.x), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], history_sample<J>(cur.y, prev.y, prev2.y), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], history_sample<J>(cur.z, prev.z, prev2.z), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], history_sample<J>(cur.w, prev.w, prev2.w), acc, 0, 0, 0);
 }
 
 template <int NG, int... Js>
 __device__ __forceinline__ void toeplitz_tile(f32x4 &acc, const float (&a)[NG][4], const f32x4 &cur,
-                                              const f32x4 &prev, std::integer_sequence<int, Js...>)
+                                              const f32x4 &prev, const f32x4 &prev2,
+                                              std::integer_sequence<int, Js...>)
 {
-    (toeplitz_step<Js>(acc, a[Js], cur, prev), ...);
+    (toeplitz_step<Js>(acc, a[Js], cur, prev, prev2), ...);
 }
 
 // Every output of lane chunk `off` in tiles t0 .. t0+nt-1 of a row, from the
@@ -366,28 +378,32 @@ __global__ __launch_bounds__(kThreads) void fir_direct_mfma_kernel(
 
     const int off = 16 * li + 4 * lq;  // the lane's chunk of a tile
     const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-    // Ring of D + 2 tiles: tile t0 + m sits in q[(m + 1) % RING].  Step m
-    // reads tiles m - 1 and m and loads tile m + D into the slot of tile
-    // m - 2; the loop is unrolled by RING so no register is copied (a copy
-    // would wait for the load in flight).
+    // Ring of D + 1 + H tiles (H = 1 history tile up to NG = 17, else 2):
+    // tile t0 + m sits in q[(m + H) % RING].  Step m reads tiles m - H .. m
+    // and loads tile m + D into the slot of tile m - H - 1; the loop is
+    // unrolled by RING so no register is copied (a copy would wait for the
+    // load in flight).
     constexpr int D = 3;
-    constexpr int RING = D + 2;
+    constexpr int H = NG > 17 ? 2 : 1;
+    constexpr int RING = D + 1 + H;
     f32x4 q[RING];
-    // The first tile's history: only lanes j >= 16 - (NG-1) of the previous
-    // tile are read by the DPP rotates (J <= NG-1), so only they load it (at
+    // The first tile's history: of tile t0 - h only lanes j >= 16h - (NG-1)
+    // are read by the DPP rotates (J <= NG-1), so only they load it (at
     // K = 33 the last 32 samples, not 256: the strip's one extra read).
     // Zeros before the row start.
-    q[0] = li >= 16 - (NG - 1) ? load4(xrow, (t0 - 1) * kMfmaTile + off, N, rv) : zero;
+#pragma unroll
+    for (int h = H; h >= 1; --h)
+        q[H - h] = li >= 16 * h - (NG - 1) ? load4(xrow, (t0 - h) * kMfmaTile + off, N, rv) : zero;
 #pragma unroll
     for (int d = 0; d < D; ++d)
-        q[d + 1] = d < nt ? load4(xrow, (t0 + d) * kMfmaTile + off, N, rv) : zero;
+        q[H + d] = d < nt ? load4(xrow, (t0 + d) * kMfmaTile + off, N, rv) : zero;
     q[RING - 1] = zero;
     // Strips wholly inside the row load and store with plain 16-byte
     // accesses: a bounds-checked load is a branch, and at its join the
     // compiler waits for every load in flight.
     const bool inside = (t0 + nt) * kMfmaTile <= N;
     bool nonfinite = false;  // a non-finite output in this lane's chunks
-    auto step = [&](auto whole, int m, f32x4 &prev, f32x4 &cur, f32x4 &slot) {
+    auto step = [&](auto whole, int m, f32x4 &prev2, f32x4 &prev, f32x4 &cur, f32x4 &slot) {
         constexpr bool W = decltype(whole)::value;
         if (W) {
             // Unconditional (a branch around a load is a join that waits):
@@ -400,7 +416,7 @@ __global__ __launch_bounds__(kThreads) void fir_direct_mfma_kernel(
             slot = load4(xrow, (t0 + m + D) * kMfmaTile + off, N, rv);
         }
         f32x4 acc = zero;
-        toeplitz_tile<NG>(acc, a, cur, prev, std::make_integer_sequence<int, NG>{});
+        toeplitz_tile<NG>(acc, a, cur, prev, prev2, std::make_integer_sequence<int, NG>{});
         float o[4] = {acc.x, acc.y, acc.z, acc.w};
         const int64_t i0 = (t0 + m) * kMfmaTile + off;
         // Past the row end o is 0 times zeros or a non-finite tap: ignored.
@@ -420,7 +436,8 @@ __global__ __launch_bounds__(kThreads) void fir_direct_mfma_kernel(
 #pragma unroll
             for (int u = 0; u < RING; ++u) {
                 if (m + u >= nt) break;
-                step(whole, m + u, q[u], q[(u + 1) % RING], q[(u + D + 1) % RING]);
+                step(whole, m + u, q[(u + H + RING - 2) % RING], q[(u + H - 1) % RING],
+                     q[(u + H) % RING], q[(u + H + D) % RING]);
             }
         }
     };
@@ -466,7 +483,9 @@ int launch_direct_mfma_k(const float *x, int64_t ldx, const float *b, int K, int
                    : launch_direct_mfma<G, false>(x, ldx, b, K, B, N, y, ldy, strip_tiles, stream);
         SCIR_NG(1) SCIR_NG(2) SCIR_NG(3) SCIR_NG(4) SCIR_NG(5) SCIR_NG(6) SCIR_NG(7) SCIR_NG(8)
         SCIR_NG(9) SCIR_NG(10) SCIR_NG(11) SCIR_NG(12) SCIR_NG(13) SCIR_NG(14) SCIR_NG(15)
-        SCIR_NG(16) SCIR_NG(17)
+        SCIR_NG(16) SCIR_NG(17) SCIR_NG(18) SCIR_NG(19) SCIR_NG(20) SCIR_NG(21) SCIR_NG(22)
+        SCIR_NG(23) SCIR_NG(24) SCIR_NG(25) SCIR_NG(26) SCIR_NG(27) SCIR_NG(28) SCIR_NG(29)
+        SCIR_NG(30) SCIR_NG(31) SCIR_NG(32) SCIR_NG(33)
 #undef SCIR_NG
     default:
         return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "FP32 MFMA direct kernel needs K <= %d",
@@ -571,12 +590,14 @@ int fir_direct_dev(const float *x, int64_t ldx, const float *b, int K, int64_t B
     // tiles); F32_EXACT and longer filters take the LDS-DMA loader's rules.
     if (tc.direct_loader == 3 && !exact && K <= kMfmaF32MaxK) {
         // Default strips, measured on [4096, 65536]: 4 tiles at K = 1 (319 us
-        // against 352 with 8), 8 at K = 17-33, 16 at K = 48, 32 from K = 64.
+        // against 352 with 8), 8 at K = 17-33, 16 at K = 48, 32 from K = 64;
+        // on [2048, 262144] 64 above K = 257 (K = 512: 5559 us, 5640 with 32).
         const int strip = tc.direct_mfma_strip > 0 ? tc.direct_mfma_strip
                           : K <= 8                 ? 4
                           : K <= 40                ? 8
                           : K <= 56                ? 16
-                                                   : 32;
+                          : K <= 257               ? 32
+                                                   : 64;
         return launch_direct_mfma_k(x, ldx, b, K, B, N, y, ldy, strip, rev, stream);
     }
     // LDS-DMA loader: forward rows, 16-byte aligned, N % 4 == 0, N < 2^29.

@@ -59,7 +59,7 @@ int guarded(const char *entry, F &&body) noexcept
 struct TuneChoice {
     int direct_r = 8;         // outputs per lane in the direct kernel
     int direct_loader = 3;    // 0: one tile per workgroup, 1: register prefetch, 2: LDS DMA,
-                              // 3: F32 mode on FP32 MFMA (K <= 257; otherwise as 2)
+                              // 3: F32 mode on FP32 MFMA (K <= 513; otherwise as 2)
     int direct_mfma_strip = 0; // 256-output tiles per wave of loader 3 (0: by K)
     int direct_seg_tiles = 4; // tiles per workgroup of the pipelined variants
     int direct_dma_bufs = 2;  // LDS buffers of the DMA loader (tiles in flight + 1), 2..4

@@ -83,7 +83,7 @@ def test_exact_bitwise_vs_reference_loop(hip, oracle, variant, K):
         assert mism.size == 0, f"K={K} N={N}: {mism.size} mismatches, first at {mism[:5]}"
 
 
-@pytest.mark.parametrize("K", [1, 5, 33, 129, 255, 1000, 2048, 3001])
+@pytest.mark.parametrize("K", [1, 5, 33, 129, 255, 400, 513, 1000, 2048, 3001])
 def test_f32_within_bound_of_fp64(hip, oracle, variant, K):
     r = rng(200 + K)
     for N in (17, 4096, 10007):
@@ -236,12 +236,13 @@ def test_unaligned_rows_equal_aligned(hip, precision, ox, oy):
 
 
 @pytest.mark.parametrize("strip", [1, 3, 8])
-@pytest.mark.parametrize("K", [16, 17, 240, 241, 256, 257])
+@pytest.mark.parametrize("K", [16, 17, 240, 241, 256, 257, 258, 497, 498, 513])
 def test_f32_mfma_boundaries_both_directions(hip, oracle, strip, K):
-    """F32 on FP32 MFMA (K <= 257) where the Toeplitz block gains a column
-    group (NG = ceil((K-1)/16) + 1 steps up after K = 16 and K = 240; K = 257
-    is the last tap count whose history is one tile back), in strips of 1, 3
-    and 8 tiles, on rows that end inside a tile, on one and past a strip:
+    """F32 on FP32 MFMA (K <= 513) where the Toeplitz block gains a column
+    group (NG = ceil((K-1)/16) + 1 steps up after K = 16, 240, 257 and 497;
+    K = 257 is the last tap count whose history is one tile back, from 498 the
+    step J = 32 reads the tile two back, 513 is the largest), in strips of 1,
+    3 and 8 tiles, on rows that end inside a tile, on one and past a strip:
     forward, and back to front as dL/dx uses it (dx[m] = sum_k b[k] g[m+k]),
     both within (K+1) 2^-23 sum|b||x| of FP64."""
     import torch

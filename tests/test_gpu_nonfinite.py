@@ -91,7 +91,7 @@ def check_positions(y, ref, what):
 
 
 WHERE = ["x", "b", "xb"]
-K_DIRECT = [1, 2, 5, 33, 129, 1000, 2049]
+K_DIRECT = [1, 2, 5, 33, 129, 300, 513, 1000, 2049]
 
 
 @pytest.mark.parametrize("where", WHERE)
