@@ -10,11 +10,15 @@ A step is one pass of the FIR over the whole batch, inputs already resident
 in HBM.  ``value`` = output samples / second over the K timed steps (device
 time, HIP events on the launch stream).  Beside it the line carries, measured
 after the timed region on the same inputs:
-  compare.fp32_direct / compare.fp16  the other arithmetic paths, Gsamples/s
-  / compare.fft32                     (fft32: FP32 FFT overlap-save, normwise error)
-                                      and rel-L2 vs the FP64 oracle
-  bandwidth_workload                  x [4096, 65536], 33-tap low-pass, FP32
-                                      direct: GB/s against the 8 TB/s HBM peak
+  compare.f32 / compare.fp16         the other arithmetic paths, Gsamples/s
+  / compare.fft32                     (f32: FP32, on the FP32 matrix cores up to
+                                      K = 513; fft32: FP32 FFT overlap-save,
+                                      normwise error) and rel-L2 vs the FP64 oracle
+  compare.fp32_direct_valu            the FP32 direct form on the vector units
+                                      (fused multiply-adds; what F32 runs above
+                                      K = 513)
+  bandwidth_workload                  x [4096, 65536], 33-tap low-pass, FP32:
+                                      GB/s against the 8 TB/s HBM peak
   roofline                            the dominant kernel of the headline:
                                       algorithmic TFLOP/s against 2.5 PFLOP/s,
                                       the issued MFMA rate (mfma_issued_frac),
@@ -361,9 +365,28 @@ def run_dense(args):
             # the switch from another kernel (measured after the FP16 runs).
             ms_o, _ = timed(lambda: device.fir(x, b, other, out=y), max(5, args.steps // 2),
                             20, stream)
-            comp["fp32_direct" if other == "f32" else other] = dict(
+            comp[other] = dict(
                 gsamples_s=round(B * N / (ms_o * 1e-3) / 1e9, 2), ms_per_step=round(ms_o, 4),
                 rel_l2_vs_fp64=rel_l2_rows(x, y, b_np, rows))
+            if other == "f32":
+                comp[other]["kernel"] = ("fir_direct_mfma_kernel (FP32 MFMA)"
+                                         if K <= F32_MFMA_MAX_K else "fir_direct_dma_kernel (FP32 VALU)")
+        # The FP32 direct form on the VALU (fused multiply-adds, the LDS-DMA
+        # loader), which F32 runs only above K = 513; below, F32 runs on the
+        # FP32 matrix cores and this is the same arithmetic class timed on
+        # the vector units.
+        if K <= F32_MFMA_MAX_K:
+            from scir_amd import _lib
+            _lib.check(_lib.lib().scir_hip_tune_set(b"direct.loader", 2))
+            try:
+                ms_o, _ = timed(lambda: device.fir(x, b, "f32", out=y), max(5, args.steps // 2),
+                                20, stream)
+            finally:
+                _lib.check(_lib.lib().scir_hip_tune_set(b"direct.loader", -1))
+            comp["fp32_direct_valu"] = dict(
+                gsamples_s=round(B * N / (ms_o * 1e-3) / 1e9, 2), ms_per_step=round(ms_o, 4),
+                rel_l2_vs_fp64=rel_l2_rows(x, y, b_np, rows),
+                kernel="fir_direct_dma_kernel (FP32 VALU)")
         out["compare"] = comp
         if name != "direct_k33":
             wl2, x2, b2, _ = dense_setup("direct_k33")
