@@ -19,8 +19,9 @@ from conftest import rng
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("precision,K", [("f32", 33), ("f32_exact", 33), ("bf16x3", 512),
-                                         ("fp16", 512), ("auto", 129), ("fft32", 700)])
+@pytest.mark.parametrize("precision,K", [("f32", 33), ("f32", 400), ("f32_exact", 33),
+                                         ("bf16x3", 512), ("fp16", 512), ("auto", 129),
+                                         ("fft32", 700)])
 def test_dense_graph_replays_match_eager(hip, precision, K):
     import torch
     from scir_amd import device
