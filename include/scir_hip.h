@@ -88,7 +88,12 @@ typedef enum scir_hip_precision {
      * (K+1) * 2^-23 * sum_k |b_k||x_{n-k}| of the exact sum like F32.
      * K <= 6689 (FP16: K <= 23457); longer filters give
      * SCIR_HIP_ERR_UNSUPPORTED.  Values beyond the bf16 (fp16) range and
-     * non-finite values take the recomputation described above. */
+     * non-finite values take the recomputation described above; so does
+     * every output of a segment of 8192 whose samples or taps include a
+     * nonzero magnitude below 2^-110 (or whose smallest sample and tap
+     * magnitudes multiply to below 2^-100), where the split terms would
+     * leave the normal range: there the bound holds with FP32's subnormal
+     * floor, (K+1) * 2^-149 added. */
     SCIR_HIP_PREC_BF16X3 = 2,
     /* Toeplitz-block contraction on MFMA with one fp16 product (x and b
      * rounded to fp16), FP32 accumulation.  Reported, not accuracy-bound. */

@@ -115,6 +115,53 @@ __device__ __forceinline__ unsigned int pack2(unsigned short lo, unsigned short 
     return (unsigned int)lo | ((unsigned int)hi << 16);
 }
 
+// Tiny magnitudes (BF16x3).  The split terms of v (hi ~ v, mid ~ 2^-8 v,
+// lo ~ 2^-16 v) are exact bf16 values, and the kept cross products exact FP32
+// values, only while they stay in the normal ranges: for samples and taps
+// whose nonzero magnitudes are at least 2^-110, with products of at least
+// 2^-100.  A segment whose staged samples or taps go below that (subnormal
+// and near-subnormal data) is recomputed from the reference loop's terms.
+// Magnitudes are tracked by binary exponent (v = m 2^e, m in [0.5, 1); a
+// zero, Inf or NaN gives 0, so it never looks tiny): 2^(e-1) <= |v| < 2^e.
+__device__ __forceinline__ int min_exponent(int m, float v)
+{
+    return min(m, __builtin_amdgcn_frexp_expf(v));
+}
+
+// Minimum over the wave: DPP row shifts, then row broadcasts; all VALU.
+__device__ __forceinline__ int wave_min(int v)
+{
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xF, 0xF, false));  // row_shr:1
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xF, 0xF, false));  // row_shr:2
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xF, 0xF, false));  // row_shr:4
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xF, 0xF, false));  // row_shr:8
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Smallest exponents of the samples (e_x) and taps (e_b) against the limits
+// above: 2^(e-1) < 2^-110, or 2^(e_x-1) 2^(e_b-1) < 2^-100.
+__device__ __forceinline__ bool tiny_magnitudes(int e_x, int e_b)
+{
+    return e_x <= -110 || e_b <= -110 || e_x + e_b <= -99;
+}
+
+// LDS after the tap tables (after the item slots, ragged): the smallest
+// sample exponent of the staged segment per wave, then the taps' per wave.
+// Each wave writes its own word after a barrier that every lane passes only
+// after reading the previous values, so the words need no reset.
+constexpr int kMinsBytes = 32;
+
+// Whether the segment just staged has tiny magnitudes (after the barrier
+// that follows staging).
+__device__ __forceinline__ bool segment_is_tiny(const int *mins)
+{
+    const int ex = min(min(mins[0], mins[1]), min(mins[2], mins[3]));
+    const int eb = min(min(mins[4], mins[5]), min(mins[6], mins[7]));
+    return tiny_magnitudes(ex, eb);
+}
+
 template <int S>
 __device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c)
 {
@@ -303,9 +350,17 @@ __device__ __forceinline__ float unit_sample(const f32x4 (&r)[2], int e)
 // Split the 8 samples of unit u into the S split arrays of the LDS.
 template <int S, int TW, bool REV>
 __device__ __forceinline__ void store_unit(unsigned char *lds, int split_stride, int u,
-                                           const f32x4 (&r)[2])
+                                           const f32x4 (&r)[2], int &min_e)
 {
     unsigned short t[S][8];
+    if (S == 3) {
+        // Smallest binary exponent of the unit's samples: one instruction per
+        // sample and a three-input minimum per two.
+#pragma unroll
+        for (int e = 0; e < 8; e += 2)
+            min_e = min(min_e, min(__builtin_amdgcn_frexp_expf(r[e >> 2][e & 3]),
+                                   __builtin_amdgcn_frexp_expf(r[e >> 2][(e & 3) + 1])));
+    }
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const float v = unit_sample<REV>(r, e);
@@ -318,6 +373,7 @@ __device__ __forceinline__ void store_unit(unsigned char *lds, int split_stride,
     for (int s = 0; s < S; ++s) {
         u32x4 w = {pack2(t[s][0], t[s][1]), pack2(t[s][2], t[s][3]), pack2(t[s][4], t[s][5]),
                    pack2(t[s][6], t[s][7])};
+
         *reinterpret_cast<u32x4 *>(lds + s * split_stride + unit_addr<TW>(u)) = w;
     }
 }
@@ -330,7 +386,7 @@ __device__ __forceinline__ void store_unit(unsigned char *lds, int split_stride,
 template <int S, int TW>
 __device__ __forceinline__ void build_tap_table(unsigned char *lds, int tab_offset,
                                                 int copy_bytes, const float *__restrict__ b,
-                                                int K, int m_hi)
+                                                int K, int m_hi, int &min_b)
 {
     const int C = 16 * (m_hi + 2 * TW + 1) + 32, L = C + 32 * TW + 32;
     constexpr int R = 4;
@@ -346,6 +402,7 @@ __device__ __forceinline__ void build_tap_table(unsigned char *lds, int tab_offs
         for (int r = 0; r < R; ++r) {
             const int jj = base + (int)threadIdx.x + r * kThreads;
             if (jj > L) break;
+            if (S == 3) min_b = min_exponent(min_b, v[r]);
             unsigned short t[3];
             if (S == 3)
                 split3(v[r], t[0], t[1], t[2]);
@@ -375,12 +432,12 @@ template <int S, int TW, int PF, bool REV>
 __device__ __forceinline__ void stage_segment(unsigned char *lds, int split_stride,
                                               const f32x4 (&raw)[PF][2],
                                               const float *__restrict__ xrow, int64_t N,
-                                              int64_t xb, int units, int tid)
+                                              int64_t xb, int units, int tid, int &min_e)
 {
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
         const int u = tid + j * kThreads;
-        if (u < units) store_unit<S, TW, REV>(lds, split_stride, u, raw[j]);
+        if (u < units) store_unit<S, TW, REV>(lds, split_stride, u, raw[j], min_e);
     }
     if (units <= PF * kThreads) return;  // K <= 2049
     for (int u0 = tid + PF * kThreads; u0 < units; u0 += kRestBatch * kThreads) {
@@ -392,7 +449,7 @@ __device__ __forceinline__ void stage_segment(unsigned char *lds, int split_stri
         for (int j = 0; j < kRestBatch; ++j) {
             const int u = u0 + j * kThreads;
             if (u >= units) break;
-            store_unit<S, TW, REV>(lds, split_stride, u, rest[j]);
+            store_unit<S, TW, REV>(lds, split_stride, u, rest[j], min_e);
         }
     }
     // Every load above has been consumed (a unit past `units` is zeros, not
@@ -430,14 +487,15 @@ __device__ __forceinline__ void contract_segment(Contraction<S, TW> &ct, const u
 }
 
 // Epilogue: tile t, lane (i, g) holds block P0 + wave*32*TW + t + TW*i,
-// samples 8q + 4g + j; outputs past N are dropped.  A non-finite output is
-// recomputed from the reference loop's terms (x row xrow, taps b[0, K)).
+// samples 8q + 4g + j; outputs past N are dropped.  A non-finite output, and
+// every output of a segment with tiny magnitudes, is recomputed from the
+// reference loop's terms (x row xrow, taps b[0, K)).
 template <int S, int TW, bool REV>
 __device__ __forceinline__ void store_segment(const Contraction<S, TW> &ct,
                                               float *__restrict__ yrow, int64_t N, int64_t P0,
                                               int wave, int i, int g,
                                               const float *__restrict__ xrow,
-                                              const float *__restrict__ b, int K)
+                                              const float *__restrict__ b, int K, bool tiny)
 {
 #pragma unroll
     for (int t = 0; t < TW; ++t) {
@@ -450,11 +508,11 @@ __device__ __forceinline__ void store_segment(const Contraction<S, TW> &ct,
             v.z = ct.acc0[t][4 * q + 2] + ct.acc1[t][4 * q + 2];
             v.w = ct.acc0[t][4 * q + 3] + ct.acc1[t][4 * q + 3];
             const int64_t n = n0 + 8 * q;
-            if (!__builtin_isfinite(v.x) || !__builtin_isfinite(v.y) ||
+            if (tiny || !__builtin_isfinite(v.x) || !__builtin_isfinite(v.y) ||
                 !__builtin_isfinite(v.z) || !__builtin_isfinite(v.w)) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    if (n + j < N && !__builtin_isfinite(v[j]))
+                    if (n + j < N && (tiny || !__builtin_isfinite(v[j])))
                         v[j] = direct_output<false>(xrow, N, REV, b, K, n + j);
             }
             if (n + 3 < N) {
@@ -514,7 +572,15 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
     prefetch(seg_first, (int)threadIdx.x);
 
     // Tap tables, built while the first segment's loads are in flight.
-    build_tap_table<S, TW>(lds, tab_offset, copy_bytes, b, K, m_hi);
+    int *mins = reinterpret_cast<int *>(lds + tab_offset + 2 * S * copy_bytes);
+    {
+        int min_b = 0;
+        build_tap_table<S, TW>(lds, tab_offset, copy_bytes, b, K, m_hi, min_b);
+        if (S == 3) {
+            min_b = wave_min(min_b);
+            if ((threadIdx.x & 63) == 0) mins[4 + (threadIdx.x >> 6)] = min_b;
+        }
+    }
 
     for (int64_t sg = seg_first; sg < seg_end; ++sg) {
         // Lane-dependent values are recomputed in every iteration rather than
@@ -530,7 +596,12 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
 
         // ---- stage: x[xb .. xb + 8*units) -> S split arrays in LDS ----
         if (sg != seg_first) __syncthreads();  // the previous segment's MFMAs are done
-        stage_segment<S, TW, PF, REV>(lds, split_stride, raw, xrow, N, xb, units, tid);
+        int min_e = 0;
+        stage_segment<S, TW, PF, REV>(lds, split_stride, raw, xrow, N, xb, units, tid, min_e);
+        if (S == 3) {
+            const int m = wave_min(min_e);
+            if (lane == 0) mins[wave] = m;
+        }
         __syncthreads();
         if (sg + 1 < seg_end) prefetch(sg + 1, tid);
 
@@ -543,7 +614,10 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_kernel(
             Contraction<S, TW> ct;
             contract_segment<S, TW>(ct, lds, split_stride, tab_offset, copy_bytes, msteps, m_hi,
                                     wave, i, g);
-            store_segment<S, TW, REV>(ct, y + row * ldy, N, P0, wave, i, g, xrow, b, K);
+            // Read after the contraction (the words change only after the
+            // next segment's first barrier): nothing extra stays live through it.
+            const bool tiny = S == 3 && segment_is_tiny(mins);
+            store_segment<S, TW, REV>(ct, y + row * ldy, N, P0, wave, i, g, xrow, b, K, tiny);
         }
     }
 }
@@ -570,6 +644,7 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_ragged_kernel(
     // is read by every lane right after that item's first barrier, and the
     // slot is written again two items later, after the next item's barrier.
     int *s_item = reinterpret_cast<int *>(lds + slot_offset);
+    int *mins = reinterpret_cast<int *>(lds + slot_offset + 16);
     int64_t tab_b0 = -1;
     int tab_K = -1;
     f32x4 raw[PF][2];
@@ -606,7 +681,12 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_ragged_kernel(
         float *__restrict__ yrow = y + w.x0;
         const int64_t seg_end = (int64_t)w.seg0 + w.nseg;
         if (w.b0 != tab_b0 || w.K != tab_K) {
-            build_tap_table<S, TW>(lds, tab_offset, copy_bytes, b + w.b0, w.K, m_hi);
+            int min_b = 0;
+            build_tap_table<S, TW>(lds, tab_offset, copy_bytes, b + w.b0, w.K, m_hi, min_b);
+            if (S == 3) {
+                min_b = wave_min(min_b);
+                if ((threadIdx.x & 63) == 0) mins[4 + (threadIdx.x >> 6)] = min_b;
+            }
             tab_b0 = w.b0;
             tab_K = w.K;
         }
@@ -618,7 +698,13 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_ragged_kernel(
             const int64_t P0 = sg * SEG_BLOCKS;
             const int64_t xb = 32 * P0 - 16 * (int64_t)m_hi;
             if (sg != w.seg0) __syncthreads();  // the previous segment's MFMAs are done
-            stage_segment<S, TW, PF, false>(lds, split_stride, raw, xrow, w.len, xb, units, tid);
+            int min_e = 0;
+            stage_segment<S, TW, PF, false>(lds, split_stride, raw, xrow, w.len, xb, units, tid,
+                                            min_e);
+            if (S == 3) {
+                const int m = wave_min(min_e);
+                if (lane == 0) mins[wave] = m;
+            }
             __syncthreads();
             if (sg + 1 < seg_end) {
                 prefetch(w, sg + 1, tid);
@@ -632,8 +718,9 @@ __global__ __launch_bounds__(kThreads) void fir_mfma_ragged_kernel(
                 Contraction<S, TW> ct;
                 contract_segment<S, TW>(ct, lds, split_stride, tab_offset, copy_bytes, msteps,
                                         m_hi, wave, i, g);
+                const bool tiny = S == 3 && segment_is_tiny(mins);
                 store_segment<S, TW, false>(ct, yrow, w.len, P0, wave, i, g, xrow, b + w.b0,
-                                            w.K);
+                                            w.K, tiny);
             }
         }
     }
@@ -647,7 +734,8 @@ size_t mfma_lds_bytes(int K)
 {
     const int m_hi = mfma_m_hi(K);
     const int units = kWaves * TW * 32 * 4 + 2 * m_hi;
-    return (size_t)S * unit_bytes(units, TW) + (size_t)2 * S * tab_copy_bytes(m_hi, TW);
+    return (size_t)S * unit_bytes(units, TW) + (size_t)2 * S * tab_copy_bytes(m_hi, TW) +
+           kMinsBytes;
 }
 
 template <int S, int TW, bool REV>
@@ -756,7 +844,7 @@ struct RaggedLayout {
         tab_offset = S * split_stride;
         copy_bytes = tab_copy_bytes(m_hi, kRaggedTW);
         slot_offset = tab_offset + 2 * S * copy_bytes;
-        bytes = (size_t)slot_offset + 16;
+        bytes = (size_t)slot_offset + 16 + kMinsBytes;
     }
 };
 

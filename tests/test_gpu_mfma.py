@@ -191,3 +191,44 @@ def test_tap_limit_and_auto_beyond_it(hip, oracle):
                 host_fir(hip, x, b, "bf16x3")
             assert e.value.status == hip.ERR_UNSUPPORTED
         assert np.all(np.abs(host_fir(hip, x, b, "auto") - y64) <= bound), K
+
+
+@pytest.mark.parametrize("case", ["x_subnormal", "x_near_min", "b_tiny", "tiny_segment"])
+def test_bf16x3_tiny_magnitudes_within_bound(hip, case):
+    """Samples or taps below the range in which the bf16 split terms and the
+    kept products stay normal (subnormal samples, samples near FLT_MIN, tiny
+    taps, one tiny stretch inside normal data): the segments that stage them
+    are recomputed from the reference loop's terms, so BF16x3 stays within
+    2 (K+1) 2^-23 sum|b||x| (plus one subnormal unit per term, the reference
+    loop's own underflow) of the reference loop (f32_exact), dense and ragged."""
+    import torch
+    from scir_amd import device
+    from scir_amd.ragged import RaggedPlan
+    print(f"bf16x3 tiny magnitudes {case}", flush=True)
+    r = rng(len(case))
+    B, N, K = 3, 20000, 200
+    x = r.standard_normal((B, N)).astype(np.float32)
+    b = (r.standard_normal(K) / np.sqrt(K)).astype(np.float32)
+    if case == "x_subnormal":
+        x *= np.float32(1e-39)
+    elif case == "x_near_min":
+        x *= np.float32(1e-36)
+    elif case == "b_tiny":
+        b *= np.float32(1e-38)
+    else:
+        x[1, 9000:9100] *= np.float32(1e-41)
+    xt, bt = torch.tensor(x, device="cuda"), torch.tensor(b, device="cuda")
+    # Compared on the host in float64 (which keeps every FP32 subnormal).
+    ye = device.fir(xt, bt, "f32_exact").cpu().numpy().astype(np.float64)
+    s = device.fir(xt.abs(), bt.abs(), "f32_exact").cpu().numpy().astype(np.float64)
+    lim = 2.0 * (K + 1) * (EPS * s + 2.0 ** -149)
+    y = device.fir(xt, bt, "bf16x3").cpu().numpy().astype(np.float64)
+    ratio = np.max(np.abs(y - ye) / lim)
+    assert ratio <= 1.0, f"dense {case}: max error / bound {ratio}"
+    offs = np.array([0, N, 2 * N, 3 * N], dtype=np.int64)
+    boffs = np.array([0, K, 2 * K, 3 * K], dtype=np.int64)
+    plan = RaggedPlan(offs, boffs, "bf16x3")
+    yr = plan.execute(xt.reshape(-1), bt.repeat(3)).cpu().numpy().astype(np.float64)
+    plan.close()
+    ratio = np.max(np.abs(yr - ye.reshape(-1)) / lim.reshape(-1))
+    assert ratio <= 1.0, f"ragged {case}: max error / bound {ratio}"
