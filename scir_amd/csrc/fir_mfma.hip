@@ -110,6 +110,33 @@ __device__ __forceinline__ void split3(float v, unsigned short &h, unsigned shor
     l = bf16_bits(bl);
 }
 
+// The same split for two values at once, as packed words (v0 in the low half,
+// v1 in the high half): one v_cvt_pk_bf16_f32 per term for both values, and
+// the words come out in the layout the split arrays store.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ unsigned int bf16x2_bits(f32x2 v)
+{
+    return __builtin_bit_cast(unsigned int, __builtin_convertvector(v, bf16x2));
+}
+
+__device__ __forceinline__ f32x2 bf16x2_widen(unsigned int w)
+{
+    return f32x2{__builtin_bit_cast(float, w << 16), __builtin_bit_cast(float, w & 0xffff0000u)};
+}
+
+__device__ __forceinline__ void split3_pair(float v0, float v1, unsigned int &h,
+                                            unsigned int &m, unsigned int &l)
+{
+    const f32x2 v = {v0, v1};
+    h = bf16x2_bits(v);
+    const f32x2 r1 = v - bf16x2_widen(h);
+    m = bf16x2_bits(r1);
+    const f32x2 r2 = r1 - bf16x2_widen(m);
+    l = bf16x2_bits(r2);
+}
+
 __device__ __forceinline__ unsigned int pack2(unsigned short lo, unsigned short hi)
 {
     return (unsigned int)lo | ((unsigned int)hi << 16);
@@ -352,7 +379,6 @@ template <int S, int TW, bool REV>
 __device__ __forceinline__ void store_unit(unsigned char *lds, int split_stride, int u,
                                            const f32x4 (&r)[2], int &min_e)
 {
-    unsigned short t[S][8];
     if (S == 3) {
         // Smallest binary exponent of the unit's samples: one instruction per
         // sample and a three-input minimum per two.
@@ -360,21 +386,24 @@ __device__ __forceinline__ void store_unit(unsigned char *lds, int split_stride,
         for (int e = 0; e < 8; e += 2)
             min_e = min(min_e, min(__builtin_amdgcn_frexp_expf(r[e >> 2][e & 3]),
                                    __builtin_amdgcn_frexp_expf(r[e >> 2][(e & 3) + 1])));
-    }
+        unsigned int w[3][4];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const float v = unit_sample<REV>(r, e);
-        if (S == 3)
-            split3(v, t[0][e], t[S > 1 ? 1 : 0][e], t[S > 2 ? 2 : 0][e]);
-        else
-            t[0][e] = f16_bits((_Float16)v);
-    }
+        for (int q = 0; q < 4; ++q)
+            split3_pair(unit_sample<REV>(r, 2 * q), unit_sample<REV>(r, 2 * q + 1), w[0][q],
+                        w[1][q], w[2][q]);
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-        u32x4 w = {pack2(t[s][0], t[s][1]), pack2(t[s][2], t[s][3]), pack2(t[s][4], t[s][5]),
-                   pack2(t[s][6], t[s][7])};
-
-        *reinterpret_cast<u32x4 *>(lds + s * split_stride + unit_addr<TW>(u)) = w;
+        for (int s = 0; s < 3; ++s)
+            *reinterpret_cast<u32x4 *>(lds + s * split_stride + unit_addr<TW>(u)) =
+                u32x4{w[s][0], w[s][1], w[s][2], w[s][3]};
+    } else {
+        unsigned short t[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) t[e] = f16_bits((_Float16)unit_sample<REV>(r, e));
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            u32x4 w = {pack2(t[0], t[1]), pack2(t[2], t[3]), pack2(t[4], t[5]), pack2(t[6], t[7])};
+            *reinterpret_cast<u32x4 *>(lds + s * split_stride + unit_addr<TW>(u)) = w;
+        }
     }
 }
 
