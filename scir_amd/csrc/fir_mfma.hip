@@ -386,11 +386,20 @@ __device__ __forceinline__ void store_unit(unsigned char *lds, int split_stride,
         for (int e = 0; e < 8; e += 2)
             min_e = min(min_e, min(__builtin_amdgcn_frexp_expf(r[e >> 2][e & 3]),
                                    __builtin_amdgcn_frexp_expf(r[e >> 2][(e & 3) + 1])));
+        // Pairs in the order the loads left them (slots 2p, 2p + 1).  Back to
+        // front, slot pair p holds logical samples 7 - 2p and 6 - 2p, so word
+        // 3 - p takes it with its halves swapped.  (Pairs formed from reversed
+        // slots made the compiler keep the prefetch registers in scratch.)
         unsigned int w[3][4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            split3_pair(unit_sample<REV>(r, 2 * q), unit_sample<REV>(r, 2 * q + 1), w[0][q],
-                        w[1][q], w[2][q]);
+        for (int p = 0; p < 4; ++p) {
+            unsigned int h, m, l;
+            split3_pair(r[p >> 1][2 * (p & 1)], r[p >> 1][2 * (p & 1) + 1], h, m, l);
+            const int q = REV ? 3 - p : p;
+            w[0][q] = REV ? __builtin_amdgcn_alignbit(h, h, 16) : h;
+            w[1][q] = REV ? __builtin_amdgcn_alignbit(m, m, 16) : m;
+            w[2][q] = REV ? __builtin_amdgcn_alignbit(l, l, 16) : l;
+        }
 #pragma unroll
         for (int s = 0; s < 3; ++s)
             *reinterpret_cast<u32x4 *>(lds + s * split_stride + unit_addr<TW>(u)) =
