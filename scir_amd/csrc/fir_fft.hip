@@ -333,20 +333,34 @@ __global__ __launch_bounds__(Plan<LOGN>::T) void fir_fft_kernel(
     if (has1 && oa + V <= N && ob + V <= N) {
         float *qa = REV ? ya + (N - 1 - oa) : ya + oa;
         float *qb = REV ? yb + (N - 1 - ob) : yb + ob;
+        // Outputs are stored as computed; a flag notes whether any came out
+        // non-finite (one class test each, no branch in the loop), and only
+        // then are the thread's outputs read back and fixed.
+        bool bad = false;
         if (accumulate) {
 #pragma unroll 4
             for (int v = tid; v < V; v += T) {
                 const float2 o = d[pad(K - 1 + v)];
                 float &pa = qa[REV ? -v : v], &pb = qb[REV ? -v : v];
-                pa = finish(pa + o.x, fa, oa + v);
-                pb = finish(pb + o.y, fb, ob + v);
+                const float a = pa + o.x, b = pb + o.y;
+                pa = a;
+                pb = b;
+                bad |= !__builtin_isfinite(a) | !__builtin_isfinite(b);
             }
         } else {
 #pragma unroll 4
             for (int v = tid; v < V; v += T) {
                 const float2 o = d[pad(K - 1 + v)];
-                qa[REV ? -v : v] = finish(o.x, fa, oa + v);
-                qb[REV ? -v : v] = finish(o.y, fb, ob + v);
+                qa[REV ? -v : v] = o.x;
+                qb[REV ? -v : v] = o.y;
+                bad |= !__builtin_isfinite(o.x) | !__builtin_isfinite(o.y);
+            }
+        }
+        if (bad) {
+            for (int v = tid; v < V; v += T) {
+                float &pa = qa[REV ? -v : v], &pb = qb[REV ? -v : v];
+                pa = finish(pa, fa, oa + v);
+                pb = finish(pb, fb, ob + v);
             }
         }
     } else {
