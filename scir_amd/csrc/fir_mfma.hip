@@ -535,6 +535,49 @@ __device__ __forceinline__ void store_segment(const Contraction<S, TW> &ct,
                                               const float *__restrict__ xrow,
                                               const float *__restrict__ b, int K, bool tiny)
 {
+    // Every output of the wave inside the row (a wave-uniform test): the
+    // stores go out unconditionally, one class test per output sets a flag,
+    // and only a flagged (or tiny) segment runs the recomputation pass over
+    // the stored values (rolled loops: indexing the accumulators there would
+    // put them in scratch).
+    if (32 * (P0 + (int64_t)(wave + 1) * 32 * TW) <= N) {
+        bool bad = false;
+#pragma unroll
+        for (int t = 0; t < TW; ++t) {
+            const int64_t n0 = 32 * (P0 + wave * 32 * TW + t + TW * i) + 4 * g;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                f32x4 v;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) v[j] = ct.acc0[t][4 * q + j] + ct.acc1[t][4 * q + j];
+                bad |= !__builtin_isfinite(v.x) | !__builtin_isfinite(v.y) |
+                       !__builtin_isfinite(v.z) | !__builtin_isfinite(v.w);
+                const int64_t n = n0 + 8 * q;
+                if (!REV)
+                    *reinterpret_cast<f32x4u *>(yrow + n) = v;
+                else
+                    *reinterpret_cast<f32x4u *>(yrow + (N - 4 - n)) = f32x4{v.w, v.z, v.y, v.x};
+            }
+        }
+        if (tiny || bad) {
+#pragma unroll 1
+            for (int t = 0; t < TW; ++t) {
+                const int64_t n0 = 32 * (P0 + wave * 32 * TW + t + TW * i) + 4 * g;
+#pragma unroll 1
+                for (int q = 0; q < 4; ++q) {
+                    const int64_t n = n0 + 8 * q;
+#pragma unroll 1
+                    for (int j = 0; j < 4; ++j) {
+                        // The thread's own store above, read back.
+                        float &y = yrow[REV ? N - 1 - (n + j) : n + j];
+                        if (tiny || !__builtin_isfinite(y))
+                            y = direct_output<false>(xrow, N, REV, b, K, n + j);
+                    }
+                }
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int t = 0; t < TW; ++t) {
         const int64_t n0 = 32 * (P0 + wave * 32 * TW + t + TW * i) + 4 * g;
