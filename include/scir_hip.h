@@ -374,6 +374,9 @@ SCIR_HIP_API int scir_hip_autotune_load(const char *cache_path);
  *   "mfma.segs"       row segments per workgroup of the MFMA kernel (>= 1); the
  *                     next segment's samples load while the current one computes
  *   "vjp.db_mfma"     1 (default): dL/db on FP32 MFMA; 0: the FP32 VALU kernel
+ *   "resample.fixed"  1 (default): up 2, down 3 with 31 taps (the reference's
+ *                     only resample_poly case) runs the register-window
+ *                     kernel; 0: the general polyphase kernel (same sums)
  *   "auto.mfma_min_k" smallest K for which AUTO picks BF16x3 MFMA (default
  *                     130: F32 on FP32 MFMA is faster below; ragged plans,
  *                     whose F32 kernel is the VALU form: 56, by the batch's

@@ -42,6 +42,7 @@ int g_auto_fft_min_k = -1;  // from the table; -1 = built-in default
 RaggedTune g_ragged;        // from the table (defaults until one is loaded)
 int g_over_auto_fft = -1, g_over_auto_fft_min_k = -1;
 int g_over_vjp_db_mfma = -1;
+int g_over_resample_fixed = -1;
 // Overrides set through scir_hip_tune_set (-1 = unset).
 int g_over_direct_r = -1, g_over_direct_loader = -1, g_over_direct_seg = -1,
     g_over_mfma_tiles = -1, g_over_auto_min_k = -1, g_over_mfma_segs = -1,
@@ -75,6 +76,7 @@ TuneChoice tune_lookup(int K, int64_t N)
     if (g_over_auto_fft >= 0) c.auto_fft = g_over_auto_fft != 0;
     if (g_over_auto_fft_min_k >= 0) c.auto_fft_min_k = g_over_auto_fft_min_k;
     if (g_over_vjp_db_mfma >= 0) c.vjp_db_mfma = g_over_vjp_db_mfma != 0;
+    if (g_over_resample_fixed >= 0) c.resample_fixed = g_over_resample_fixed != 0;
     if (g_over_direct_r >= 0) c.direct_r = g_over_direct_r;
     if (g_over_direct_loader >= 0) c.direct_loader = g_over_direct_loader;
     if (g_over_direct_seg >= 0) c.direct_seg_tiles = g_over_direct_seg;
@@ -186,6 +188,9 @@ extern "C" int scir_hip_tune_set(const char *key, int32_t value)
         } else if (!std::strcmp(key, "vjp.db_mfma")) {
             if (v > 1) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "vjp.db_mfma must be 0 or 1");
             g_over_vjp_db_mfma = v;
+        } else if (!std::strcmp(key, "resample.fixed")) {
+            if (v > 1) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "resample.fixed must be 0 or 1");
+            g_over_resample_fixed = v;
         } else if (!std::strcmp(key, "mfma.segs")) {
             if (v == 0) return fail(SCIR_HIP_ERR_INVALID_ARGUMENT, "mfma.segs must be >= 1");
             g_over_mfma_segs = v;

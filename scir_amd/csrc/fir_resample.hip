@@ -13,7 +13,8 @@
 // One workgroup computes T = 256*J consecutive outputs of one row; lane l
 // owns outputs m0 + l + 256*j (consecutive lanes, consecutive outputs: the
 // stores coalesce).  The x samples the tile needs and the taps are staged in
-// LDS once.
+// LDS once.  The reference's own case (up 2, down 3, 31 taps) has a kernel
+// that keeps each lane's sample window in registers (resample_fixed_kernel).
 #include "common.hpp"
 
 #include <algorithm>
@@ -104,6 +105,104 @@ __global__ __launch_bounds__(kThreads) void resample_poly_kernel(
     }
 }
 
+// The reference's only resample_poly case (up 2, down 3, its 31-tap filter;
+// crates/scir-signal/src/lib.rs:313-345), any 31 taps: lane l owns the J = 8
+// consecutive outputs m0 + 8l + j.  Their taps reach back over one window of
+// WIN samples starting STEP*l samples into the tile, so the lane reads that
+// window from LDS into registers once (WIN reads instead of one per term) and
+// every term indexes it at a compile-time offset.  The taps are uniform
+// (scalar loads).  Each output sums its terms in the general kernel's order
+// (t ascending, FMA from 0), so the results are the same bits.  Row i of the
+// LDS samples sits at i + i / STEP: lanes STEP samples apart then hit
+// different banks.
+template <int UP, int DOWN, int K>
+struct FixedRatio {
+    static constexpr int J = 8;
+    static constexpr int TMAX = (K + UP - 1) / UP;
+    static constexpr int STEP = DOWN * J / UP;
+    static constexpr int WIN = TMAX + (UP - 1 + DOWN * (J - 1)) / UP;
+    static_assert((DOWN * J) % UP == 0, "a lane's first output keeps the tile's phase");
+    static_assert(UP <= 2, "phase dispatch below covers up <= 2");
+    static_assert(kThreads * J == kT, "same tile as the general kernel");
+};
+
+template <int UP, int DOWN, int K, int R>
+__device__ __forceinline__ void fixed_ratio_outputs(const double (&win)[FixedRatio<UP, DOWN, K>::WIN],
+                                                    const double (&hk)[K], double (&acc)[8])
+{
+    using F = FixedRatio<UP, DOWN, K>;
+#pragma unroll
+    for (int j = 0; j < F::J; ++j) {
+        const int nl = R + DOWN * j;  // compile-time after unrolling
+        const int ph = nl % UP, o = nl / UP;
+        double a = 0.0;
+#pragma unroll
+        for (int t = 0; t < F::TMAX; ++t)
+            if (ph + UP * t < K) a = __builtin_fma(hk[ph + UP * t], win[F::TMAX - 1 + o - t], a);
+        acc[j] = a;
+    }
+}
+
+template <int UP, int DOWN, int K>
+__global__ __launch_bounds__(kThreads) void resample_fixed_kernel(
+    const double *__restrict__ x, int64_t ldx, int64_t N, double *__restrict__ y, int64_t ldy,
+    int64_t M, const double *__restrict__ h, int tiles_per_row, int span)
+{
+    using F = FixedRatio<UP, DOWN, K>;
+    extern __shared__ __attribute__((aligned(16))) double xs[];  // sample i at i + i / STEP
+    const int64_t row = blockIdx.x / tiles_per_row;
+    const int64_t m0 = (int64_t)(blockIdx.x % tiles_per_row) * kT;
+    constexpr int64_t off = (K - 1) / 2;
+    const int64_t n0 = off + DOWN * m0;
+    const int64_t s0 = n0 / UP - (F::TMAX - 1);
+    const int r0 = (int)(n0 % UP);
+    const double *__restrict__ xr = x + row * ldx;
+    for (int i = 2 * threadIdx.x; i < span; i += 2 * kThreads) {
+        const int64_t s = s0 + i;
+        double a, b;
+        if (s >= 0 && s + 1 < N) {
+            const f64x2u v = *reinterpret_cast<const f64x2u *>(xr + s);
+            a = v.x;
+            b = v.y;
+        } else {
+            a = (s >= 0 && s < N) ? xr[s] : 0.0;
+            b = (s + 1 >= 0 && s + 1 < N) ? xr[s + 1] : 0.0;
+        }
+        xs[i + i / F::STEP] = a;
+        if (i + 1 < span) xs[i + 1 + (i + 1) / F::STEP] = b;
+    }
+    double hk[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) hk[k] = h[k];
+    __syncthreads();
+    const int l = threadIdx.x;
+    // Window sample w = LDS sample STEP*l + w, at STEP*l + w + l + w / STEP.
+    double win[F::WIN];
+#pragma unroll
+    for (int w = 0; w < F::WIN; ++w) win[w] = xs[(F::STEP + 1) * l + w + w / F::STEP];
+    double acc[F::J];
+    // The tile's phase r0 (uniform) is the phase of every lane's first output.
+    if (r0 == 0)
+        fixed_ratio_outputs<UP, DOWN, K, 0>(win, hk, acc);
+    else
+        fixed_ratio_outputs<UP, DOWN, K, UP - 1>(win, hk, acc);
+    // Outputs through LDS (output d at d + d / 32), then stored as the general
+    // kernel stores them: consecutive lanes, consecutive outputs.
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < F::J; ++j) {
+        const int d = F::J * l + j;
+        xs[d + d / 32] = acc[j];
+    }
+    __syncthreads();
+    double *__restrict__ yr = y + row * ldy;
+#pragma unroll
+    for (int j = 0; j < F::J; ++j) {
+        const int d = l + kThreads * j;
+        if (m0 + d < M) __builtin_nontemporal_store(xs[d + d / 32], yr + m0 + d);
+    }
+}
+
 }  // namespace
 }  // namespace scir
 
@@ -126,6 +225,13 @@ int resample_launch(const double *x, int64_t ldx, int64_t B, int64_t N, int32_t 
                                160 * 1024));
     SCIR_TRY(allow_dynamic_lds(reinterpret_cast<const void *>(&resample_poly_kernel<false>),
                                160 * 1024));
+    using Fixed = FixedRatio<2, 3, 31>;
+    const bool fixed = up == 2 && down == 3 && K == 31 && tune_lookup(K, N).resample_fixed;
+    if (fixed && (int64_t)Fixed::STEP * (kThreads - 1) + Fixed::WIN > span)
+        return fail(SCIR_HIP_ERR_BACKEND, "resample tile span %d below the fixed kernel's window",
+                    span);
+    const size_t lds_fixed =
+        std::max((size_t)span + span / Fixed::STEP + 1, (size_t)kT + kT / 32) * sizeof(double);
     // Every output of a lane has one phase when up divides kThreads * down.
     const bool uni = ((int64_t)kThreads * down) % up == 0;
     const int64_t tiles = (M + kT - 1) / kT;
@@ -133,7 +239,11 @@ int resample_launch(const double *x, int64_t ldx, int64_t B, int64_t N, int32_t 
     const int64_t rows_per_launch = std::max<int64_t>(1, 0x7fffffff / tiles);
     for (int64_t r0 = 0; r0 < B; r0 += rows_per_launch) {
         const int64_t rows = std::min(rows_per_launch, B - r0);
-        if (uni)
+        if (fixed)
+            hipLaunchKernelGGL((resample_fixed_kernel<2, 3, 31>), dim3((unsigned)(rows * tiles)),
+                               dim3(kThreads), lds_fixed, s, x + r0 * ldx, ldx, N, y + r0 * ldy,
+                               ldy, M, h, (int)tiles, span);
+        else if (uni)
             hipLaunchKernelGGL(resample_poly_kernel<true>, dim3((unsigned)(rows * tiles)),
                                dim3(kThreads), lds, s, x + r0 * ldx, ldx, N, y + r0 * ldy, ldy, M,
                                h, K, up, down, (int)tiles, span);
@@ -141,7 +251,7 @@ int resample_launch(const double *x, int64_t ldx, int64_t B, int64_t N, int32_t 
             hipLaunchKernelGGL(resample_poly_kernel<false>, dim3((unsigned)(rows * tiles)),
                                dim3(kThreads), lds, s, x + r0 * ldx, ldx, N, y + r0 * ldy, ldy, M,
                                h, K, up, down, (int)tiles, span);
-        SCIR_TRY(check_launch("resample_poly_kernel"));
+        SCIR_TRY(check_launch(fixed ? "resample_fixed_kernel" : "resample_poly_kernel"));
     }
     return SCIR_HIP_OK;
 }

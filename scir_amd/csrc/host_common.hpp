@@ -72,6 +72,7 @@ struct TuneChoice {
     // when enabled with the tuning key auto.fft = 1 (off by default).
     bool auto_fft = false;
     bool vjp_db_mfma = true;  // dL/db on FP32 MFMA (false: the FP32 VALU kernel)
+    bool resample_fixed = true;  // up 2, down 3, 31 taps: the register-window kernel
     int auto_fft_min_k = 512; // measured: FFT32 beats BF16x3 MFMA from here (K <= 8192)
 };
 

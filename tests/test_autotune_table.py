@@ -57,6 +57,9 @@ def test_overrides_validated():
     assert L.scir_hip_tune_set(b"vjp.db_mfma", 2) == _lib.ERR_INVALID_ARGUMENT
     assert L.scir_hip_tune_set(b"vjp.db_mfma", 0) == _lib.OK
     assert L.scir_hip_tune_set(b"vjp.db_mfma", -1) == _lib.OK
+    assert L.scir_hip_tune_set(b"resample.fixed", 2) == _lib.ERR_INVALID_ARGUMENT
+    assert L.scir_hip_tune_set(b"resample.fixed", 0) == _lib.OK
+    assert L.scir_hip_tune_set(b"resample.fixed", -1) == _lib.OK
     assert L.scir_hip_tune_set(b"no.such.key", 1) == _lib.ERR_INVALID_ARGUMENT
     assert L.scir_hip_tune_set(b"direct.r", 16) == _lib.OK
     assert L.scir_hip_tune_set(b"direct.r", -1) == _lib.OK

@@ -102,3 +102,24 @@ def test_unaligned_rows_equal_aligned(hip, ox, oy):
     assert torch.equal(ya, ybase[:, oy:oy + M])
     yb = ybase.cpu().numpy()
     assert np.isnan(yb[:, :oy]).all() and np.isnan(yb[:, oy + M:]).all()
+
+
+@pytest.mark.parametrize("B,N", [(3, 1), (3, 2), (3, 3), (5, 100), (4, 5003), (2, 70001)])
+def test_fixed_ratio_kernel_equals_general(hip, B, N):
+    """up 2, down 3 with 31 taps (the reference's one case) runs the
+    register-window kernel; the general kernel (tuning key resample.fixed = 0)
+    sums the same terms in the same order: the same bits, row tails included."""
+    from scir_amd import _lib
+    print(f"resample fixed vs general B={B} N={N}", flush=True)
+    r = rng(N + 7)
+    x = r.standard_normal((B, N))
+    x[0, ::13] *= 1e3
+    h = r.standard_normal(31)
+    y_fixed = signal.resample_poly_batched(x, 2, 3, h, Device.Hip)
+    _lib.check(_lib.lib().scir_hip_tune_set(b"resample.fixed", 0))
+    try:
+        y_general = signal.resample_poly_batched(x, 2, 3, h, Device.Hip)
+    finally:
+        _lib.check(_lib.lib().scir_hip_tune_set(b"resample.fixed", -1))
+    assert y_fixed.shape == y_general.shape
+    assert np.array_equal(y_fixed, y_general)
