@@ -665,7 +665,8 @@ def run_resample(args):
                wall_s=round(wall, 3),
                roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                              frac=round(gbs / HBM_PEAK_GBS, 4),
-                             traffic=load_traffic("resample_poly_f64_resample_2_3_pmc")))
+                             traffic=load_traffic("resample_poly_f64_resample_2_3_pmc"),
+                             kernel="resample_fixed_kernel<2,3,31> (lane windows in registers)"))
     rows = [0, B - 1]
     from oracle import oracle as orc
     xs = x[rows].cpu().numpy()
