@@ -112,8 +112,8 @@ __global__ __launch_bounds__(kThreads) void resample_poly_kernel(
 // window from LDS into registers once (WIN reads instead of one per term) and
 // every term indexes it at a compile-time offset.  The taps are uniform
 // (scalar loads).  Each output sums its terms in the general kernel's order
-// (t ascending, FMA from 0), so the results are the same bits.  Row i of the
-// LDS samples sits at i + i / STEP: lanes STEP samples apart then hit
+// (t ascending, FMA from 0), so the results are the same bits.  Sample i of
+// the LDS copy sits at i + i / STEP: lanes STEP samples apart then hit
 // different banks.
 template <int UP, int DOWN, int K>
 struct FixedRatio {
