@@ -85,3 +85,12 @@ def test_error_message_is_thread_local():
     t.start(); t.join()
     assert seen == [b""]
     assert L.scir_hip_last_error() != b""
+
+
+def test_every_tuning_key_is_documented():
+    """scir_hip_tune_set accepts exactly the keys the header lists."""
+    src = open(os.path.join(ROOT, "scir_amd", "csrc", "autotune.hip")).read()
+    accepted = set(re.findall(r'strcmp\(key, "([a-z_.0-9]+)"\)', src))
+    documented = set(re.findall(r'^ \*   "([a-z_.0-9]+)"', open(HEADER).read(), re.M))
+    assert len(accepted) >= 15
+    assert accepted == documented
